@@ -1,0 +1,89 @@
+"""Loader for libcuszp_amd.so (the HIP kernels behind the C ABI of include/cuSZp.h).
+
+There is no fallback: if the library is missing, importing the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcuszp_amd.so")
+
+_lib = None
+
+
+class Uint3(ctypes.Structure):
+    """cuszp_dims_t / CUDA uint3: {x, y, z}, x fastest (python/pycuSZp.py:8-9 of the reference)."""
+
+    _fields_ = [("x", ctypes.c_uint), ("y", ctypes.c_uint), ("z", ctypes.c_uint)]
+
+
+EXPORTED = (
+    ["cuSZp_compress", "cuSZp_decompress"]
+    + [
+        f"cuSZp_{d}_{dim}_{m}_{t}"
+        for t in ("f32", "f64")
+        for dim in ("1D", "2D", "3D")
+        for m in ("fixed", "plain", "outlier")
+        for d in ("compress", "decompress")
+    ]
+    + [
+        "cuSZp_max_compressed_size",
+        "cuSZp_compress_ex",
+        "cuSZp_decompress_ex",
+        "cuSZp_compress_async",
+        "cuSZp_decompress_async",
+        "cuSZp_last_async_status",
+        "cuSZp_status_string",
+        "cuSZp_version",
+    ]
+)
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python cuszp_amd/build.py`"
+        )
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i32, f32, f64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_float, ctypes.c_double
+    psz = ctypes.POINTER(ctypes.c_size_t)
+    # generic entries: include/cuSZp.h:38-45 of the reference
+    L.cuSZp_compress.argtypes = [vp, vp, sz, psz, f32, i32, Uint3, i32, i32, vp]
+    L.cuSZp_compress.restype = None
+    L.cuSZp_decompress.argtypes = [vp, vp, sz, sz, f32, i32, Uint3, i32, i32, vp]
+    L.cuSZp_decompress.restype = None
+    for t, ft in (("f32", f32), ("f64", f64)):
+        for m in ("fixed", "plain", "outlier"):
+            c = getattr(L, f"cuSZp_compress_1D_{m}_{t}")
+            c.argtypes, c.restype = [vp, vp, sz, psz, ft, vp], None
+            d = getattr(L, f"cuSZp_decompress_1D_{m}_{t}")
+            d.argtypes, d.restype = [vp, vp, sz, sz, ft, vp], None
+            for dim in ("2D", "3D"):
+                c = getattr(L, f"cuSZp_compress_{dim}_{m}_{t}")
+                c.argtypes, c.restype = [vp, vp, sz, psz, Uint3, ft, vp], None
+                d = getattr(L, f"cuSZp_decompress_{dim}_{m}_{t}")
+                d.argtypes, d.restype = [vp, vp, sz, sz, Uint3, ft, vp], None
+    L.cuSZp_max_compressed_size.argtypes = [sz, i32, Uint3, i32, i32]
+    L.cuSZp_max_compressed_size.restype = sz
+    L.cuSZp_compress_ex.argtypes = [vp, vp, sz, sz, psz, f64, i32, Uint3, i32, i32, vp]
+    L.cuSZp_compress_ex.restype = i32
+    L.cuSZp_decompress_ex.argtypes = [vp, vp, sz, sz, f64, i32, Uint3, i32, i32, vp]
+    L.cuSZp_decompress_ex.restype = i32
+    L.cuSZp_compress_async.argtypes = [vp, vp, sz, sz, vp, f64, i32, Uint3, i32, i32, vp]
+    L.cuSZp_compress_async.restype = i32
+    L.cuSZp_decompress_async.argtypes = [vp, vp, sz, sz, f64, i32, Uint3, i32, i32, vp]
+    L.cuSZp_decompress_async.restype = i32
+    L.cuSZp_last_async_status.argtypes = [vp]
+    L.cuSZp_last_async_status.restype = i32
+    L.cuSZp_status_string.argtypes = [i32]
+    L.cuSZp_status_string.restype = ctypes.c_char_p
+    L.cuSZp_version.argtypes = []
+    L.cuSZp_version.restype = ctypes.c_char_p
+    _lib = L
+    return L

@@ -1,0 +1,357 @@
+// C ABI of the MI355X fused compressor: argument validation, geometry, the
+// per-(device, stream) scan-state pool and the blocking / stream-ordered entry
+// points declared in include/cuSZp.h.
+//
+// Reference behaviour mirrored:
+//   generic dispatch .......... src/cuSZp.cu:4-165 (float eb widened for f64, :45/:127)
+//   per-dimension entries ..... src/cuSZp_entry_{1,2,3}D_f{32,64}.cu
+//   launch geometry / sizes ... src/cuSZp_entry_1D_f32.cu:20-43, src/cuSZp_entry_2D_f32.cu:21-47,
+//                               src/cuSZp_entry_3D_f32.cu:21-48
+// What differs: scratch comes from a persistent pool instead of three
+// cudaMalloc/cudaMemset/cudaFree per call, the size readback goes through
+// pinned memory with a stream-local synchronisation, and every argument is
+// checked before anything is launched.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <utility>
+
+#include "../../include/cuSZp.h"
+#include "launch.hpp"
+
+namespace cuszp {
+namespace {
+
+constexpr uint64_t kMaxStream = (1ull << 38) - 1;   // status-word value field
+
+struct Slot {
+    std::mutex mu;
+    unsigned long long* d_status = nullptr;   // tile status words
+    uint64_t cap_tiles = 0;
+    unsigned long long* d_misc = nullptr;     // [0] ticket, [1] total, [2] error bits
+    unsigned long long* h_res = nullptr;      // pinned [2]
+    uint32_t epoch = 0;
+};
+
+std::mutex g_pool_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<Slot>> g_pool;
+
+Slot& slot_for(int dev, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto& p = g_pool[{dev, st}];
+    if (!p) p.reset(new Slot());
+    return *p;
+}
+
+#define CUSZP_HIP(call)                                \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return CUSZP_ERROR_HIP;  \
+    } while (0)
+
+// Make room for `ntiles` status words and advance the epoch. Caller holds s.mu.
+int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
+    if (!s.d_misc) {
+        CUSZP_HIP(hipMalloc((void**)&s.d_misc, 4 * sizeof(unsigned long long)));
+        CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, 4 * sizeof(unsigned long long), st));
+        CUSZP_HIP(hipHostMalloc((void**)&s.h_res, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+    }
+    if (ntiles > s.cap_tiles) {
+        if (s.d_status) {
+            CUSZP_HIP(hipStreamSynchronize(st));
+            CUSZP_HIP(hipFree(s.d_status));
+            s.d_status = nullptr;
+        }
+        uint64_t cap = s.cap_tiles * 2;
+        if (cap < ntiles) cap = ntiles;
+        if (cap < 4096) cap = 4096;
+        CUSZP_HIP(hipMalloc((void**)&s.d_status, cap * sizeof(unsigned long long)));
+        CUSZP_HIP(hipMemsetAsync(s.d_status, 0, cap * sizeof(unsigned long long), st));
+        s.cap_tiles = cap;
+        s.epoch = 0;
+    }
+    if (++s.epoch >= (1u << 24)) {   // 24-bit epoch tag wrapped: clear stale words
+        CUSZP_HIP(hipMemsetAsync(s.d_status, 0, s.cap_tiles * sizeof(unsigned long long), st));
+        s.epoch = 1;
+    }
+    return CUSZP_SUCCESS;
+}
+
+ScanArgs scan_args(const Slot& s) {
+    ScanArgs a;
+    a.status = s.d_status;
+    a.ticket = s.d_misc;
+    a.result = s.d_misc + 1;
+    a.tag = (unsigned long long)s.epoch << kEpochShift;
+    return a;
+}
+
+// Geometry of a (dim, nbEle, dims) triple; mirrors the reference's block counts.
+int make_geom(int dim, size_t n, cuszp_dims_t d, Geom& g) {
+    std::memset(&g, 0, sizeof(g));
+    if (dim == CUSZP_DIM_1D) {
+        g.n = n;
+        g.nblocks = (n + 31) / 32;
+        g.H = (n + 32767) / 32768 * 1024;          // src/cuSZp_entry_1D_f32.cu:43
+        g.ntiles = g.H / 2 / kGroupsPerTile;       // 2 header bytes per group
+        return CUSZP_SUCCESS;
+    }
+    if (dim != CUSZP_DIM_2D && dim != CUSZP_DIM_3D) return CUSZP_ERROR_INVALID_ARGUMENT;
+    if (!d.x || !d.y || !d.z || (d.x % 4)) return CUSZP_ERROR_INVALID_DIMS;   // float4 loads in the reference
+    const uint64_t xyz = (uint64_t)d.x * d.y * d.z;
+    if ((uint64_t)n != xyz) return CUSZP_ERROR_INVALID_DIMS;
+    g.n = n;
+    g.X = d.x; g.Y = d.y; g.Z = d.z;
+    g.XY = (uint64_t)d.x * d.y;
+    uint64_t bz;
+    if (dim == CUSZP_DIM_2D) {                      // src/cuSZp_entry_2D_f32.cu:21-23
+        g.Bx = (d.x + 7) / 8; g.By = (d.y + 7) / 8; bz = d.z;
+    } else {                                        // src/cuSZp_entry_3D_f32.cu:21-24
+        g.Bx = (d.x + 3) / 4; g.By = (d.y + 3) / 4; bz = (d.z + 3) / 4;
+    }
+    g.nblocks = (uint64_t)g.Bx * g.By * bz;
+    if (g.nblocks >= (1ull << 40)) return CUSZP_ERROR_TOO_LARGE;
+    g.H = (g.nblocks + 3) / 4 * 4;                  // src/cuSZp_kernels_2D_f32.cu:46
+    g.ntiles = (g.nblocks + kGroupsPerTile - 1) / kGroupsPerTile;
+    return CUSZP_SUCCESS;
+}
+
+uint64_t max_stream_bytes(int dim, const Geom& g) {
+    const uint64_t S = (dim == CUSZP_DIM_1D) ? 4 : 8;
+    return g.H + g.nblocks * (S + 32 * S + 4);
+}
+
+bool bad_enums(int dim, int type, int mode) {
+    return (dim < 1 || dim > 3) || (type != CUSZP_TYPE_FLOAT && type != CUSZP_TYPE_DOUBLE) ||
+           (mode < CUSZP_MODE_FIXED || mode > CUSZP_MODE_OUTLIER);
+}
+
+bool bad_eb(double eb, int type) {
+    if (type == CUSZP_TYPE_FLOAT) {
+        const float f = (float)eb;
+        return !(f > 0.0f) || !std::isfinite(f);
+    }
+    return !(eb > 0.0) || !std::isfinite(eb);
+}
+
+// Async error bits accumulated per slot for cuSZp_last_async_status.
+int status_from_bits(unsigned long long bits) {
+    if (bits & kErrOverflow) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
+    if (bits & kErrTruncated) return CUSZP_ERROR_CORRUPT_STREAM;
+    return CUSZP_SUCCESS;
+}
+
+int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
+                  unsigned long long* d_size, double eb, int dim, cuszp_dims_t dims, int type, int mode,
+                  hipStream_t st, bool blocking) {
+    if (cmp_size) *cmp_size = 0;
+    if (bad_enums(dim, type, mode) || bad_eb(eb, type)) return CUSZP_ERROR_INVALID_ARGUMENT;
+    Geom g;
+    int rc = make_geom(dim, n, dims, g);
+    if (rc) return rc;
+    if (n == 0) {
+        if (d_size) CUSZP_HIP(hipMemsetAsync(d_size, 0, sizeof(*d_size), st));
+        return CUSZP_SUCCESS;
+    }
+    if (!in || !out) return CUSZP_ERROR_INVALID_ARGUMENT;
+    if (max_stream_bytes(dim, g) > kMaxStream) return CUSZP_ERROR_TOO_LARGE;
+    if (cap < g.H) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
+    int dev = 0;
+    CUSZP_HIP(hipGetDevice(&dev));
+    Slot& s = slot_for(dev, st);
+    std::lock_guard<std::mutex> lk(s.mu);
+    rc = prepare(s, g.ntiles, st);
+    if (rc) return rc;
+    CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, (blocking ? 3 : 2) * sizeof(unsigned long long), st));
+    const ScanArgs sa = scan_args(s);
+    hipError_t e;
+    if (type == CUSZP_TYPE_FLOAT)
+        e = launch_compress_f32(dim, mode, (const float*)in, out, g, (float)eb, sa, cap, st);
+    else
+        e = launch_compress_f64(dim, mode, (const double*)in, out, g, eb, sa, cap, st);
+    if (e != hipSuccess) return CUSZP_ERROR_HIP;
+    if (d_size) CUSZP_HIP(hipMemcpyAsync(d_size, s.d_misc + 1, sizeof(*d_size), hipMemcpyDeviceToDevice, st));
+    if (!blocking) return CUSZP_SUCCESS;
+    CUSZP_HIP(hipMemcpyAsync(s.h_res, s.d_misc + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    CUSZP_HIP(hipStreamSynchronize(st));
+    rc = status_from_bits(s.h_res[1]);
+    if (cmp_size) *cmp_size = (size_t)s.h_res[0];
+    return rc;
+}
+
+int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_size, double eb, int dim,
+                    cuszp_dims_t dims, int type, int mode, hipStream_t st, bool blocking) {
+    if (bad_enums(dim, type, mode) || bad_eb(eb, type)) return CUSZP_ERROR_INVALID_ARGUMENT;
+    Geom g;
+    int rc = make_geom(dim, n, dims, g);
+    if (rc) return rc;
+    if (n == 0) return CUSZP_SUCCESS;
+    if (!out || !cmp) return CUSZP_ERROR_INVALID_ARGUMENT;
+    if (max_stream_bytes(dim, g) > kMaxStream) return CUSZP_ERROR_TOO_LARGE;
+    if (cmp_size < g.H) return CUSZP_ERROR_CORRUPT_STREAM;
+    int dev = 0;
+    CUSZP_HIP(hipGetDevice(&dev));
+    Slot& s = slot_for(dev, st);
+    std::lock_guard<std::mutex> lk(s.mu);
+    rc = prepare(s, g.ntiles, st);
+    if (rc) return rc;
+    CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, (blocking ? 3 : 2) * sizeof(unsigned long long), st));
+    const ScanArgs sa = scan_args(s);
+    hipError_t e;
+    if (type == CUSZP_TYPE_FLOAT)
+        e = launch_decompress_f32(dim, mode, (float*)out, cmp, g, (float)eb, sa, cmp_size, st);
+    else
+        e = launch_decompress_f64(dim, mode, (double*)out, cmp, g, eb, sa, cmp_size, st);
+    if (e != hipSuccess) return CUSZP_ERROR_HIP;
+    if (!blocking) return CUSZP_SUCCESS;
+    CUSZP_HIP(hipMemcpyAsync(s.h_res, s.d_misc + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    CUSZP_HIP(hipStreamSynchronize(st));
+    return status_from_bits(s.h_res[1]);
+}
+
+void report(const char* fn, int rc) {
+    if (rc != CUSZP_SUCCESS) std::fprintf(stderr, "cuSZp: %s: %s\n", fn, cuSZp_status_string(rc));
+}
+
+const cuszp_dims_t kNoDims = {0, 0, 0};
+
+}  // namespace
+}  // namespace cuszp
+
+using namespace cuszp;
+
+extern "C" {
+
+const char* cuSZp_status_string(int status) {
+    switch (status) {
+        case CUSZP_SUCCESS: return "success";
+        case CUSZP_ERROR_INVALID_ARGUMENT: return "invalid argument (enum, pointer or error bound)";
+        case CUSZP_ERROR_INVALID_DIMS: return "invalid dims (need x % 4 == 0, non-zero extents, nbEle == x*y*z)";
+        case CUSZP_ERROR_TOO_LARGE: return "input too large for the stream format";
+        case CUSZP_ERROR_OUTPUT_TOO_SMALL: return "compressed output exceeds the buffer capacity";
+        case CUSZP_ERROR_CORRUPT_STREAM: return "compressed stream is shorter than its header describes";
+        case CUSZP_ERROR_HIP: return "HIP runtime error";
+        default: return "unknown status";
+    }
+}
+
+const char* cuSZp_version(void) { return "cuszp_amd 0.1.0 gfx950"; }
+
+size_t cuSZp_max_compressed_size(size_t nbEle, cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type,
+                                 cuszp_mode_t mode) {
+    if (bad_enums(dim, type, mode)) return 0;
+    Geom g;
+    if (make_geom(dim, nbEle, dims, g)) return 0;
+    if (nbEle == 0) return 0;
+    return (size_t)max_stream_bytes(dim, g);
+}
+
+int cuSZp_compress_ex(const void* d_oriData, unsigned char* d_cmpBytes, size_t cmpCapacity, size_t nbEle,
+                      size_t* cmpSize, double errorBound, cuszp_dim_t dim, cuszp_dims_t dims,
+                      cuszp_type_t type, cuszp_mode_t mode, hipStream_t stream) {
+    return compress_impl(d_oriData, d_cmpBytes, cmpCapacity, nbEle, cmpSize, nullptr, errorBound, dim, dims,
+                         type, mode, stream, true);
+}
+
+int cuSZp_decompress_ex(void* d_decData, const unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize,
+                        double errorBound, cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type,
+                        cuszp_mode_t mode, hipStream_t stream) {
+    return decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, errorBound, dim, dims, type, mode, stream,
+                           true);
+}
+
+int cuSZp_compress_async(const void* d_oriData, unsigned char* d_cmpBytes, size_t cmpCapacity, size_t nbEle,
+                         unsigned long long* d_cmpSize, double errorBound, cuszp_dim_t dim, cuszp_dims_t dims,
+                         cuszp_type_t type, cuszp_mode_t mode, hipStream_t stream) {
+    return compress_impl(d_oriData, d_cmpBytes, cmpCapacity, nbEle, nullptr, d_cmpSize, errorBound, dim, dims,
+                         type, mode, stream, false);
+}
+
+int cuSZp_decompress_async(void* d_decData, const unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize,
+                           double errorBound, cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type,
+                           cuszp_mode_t mode, hipStream_t stream) {
+    return decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, errorBound, dim, dims, type, mode, stream,
+                           false);
+}
+
+int cuSZp_last_async_status(hipStream_t stream) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CUSZP_ERROR_HIP;
+    Slot& s = slot_for(dev, stream);
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (!s.d_misc) return CUSZP_SUCCESS;
+    if (hipMemcpyAsync(s.h_res, s.d_misc + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return CUSZP_ERROR_HIP;
+    const int rc = status_from_bits(s.h_res[1]);
+    if (hipMemsetAsync(s.d_misc + 2, 0, sizeof(unsigned long long), stream) != hipSuccess) return CUSZP_ERROR_HIP;
+    return rc;
+}
+
+// ---- generic drop-ins: src/cuSZp.cu:4-83, :86-165 ------------------------
+void cuSZp_compress(void* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, float errorBound,
+                    cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode, hipStream_t stream) {
+    // f64 receives the float error bound widened to double (src/cuSZp.cu:45)
+    report("cuSZp_compress", compress_impl(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr,
+                                           (double)errorBound, dim, dims, type, mode, stream, true));
+}
+
+void cuSZp_decompress(void* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, float errorBound,
+                      cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
+                      hipStream_t stream) {
+    report("cuSZp_decompress", decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, dim,
+                                               dims, type, mode, stream, true));
+}
+
+// ---- per-dimension drop-ins: src/cuSZp_entry_{1,2,3}D_f{32,64}.cu --------
+#define CUSZP_ENTRY_1D(MODE_NAME, MODE_ENUM, T, TNAME, TENUM)                                               \
+    void cuSZp_compress_1D_##MODE_NAME##_##TNAME(T* d_oriData, unsigned char* d_cmpBytes, size_t nbEle,     \
+                                                  size_t* cmpSize, T errorBound, hipStream_t stream) {      \
+        report("cuSZp_compress_1D_" #MODE_NAME "_" #TNAME,                                                  \
+               compress_impl(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr, (double)errorBound, \
+                             CUSZP_DIM_1D, kNoDims, TENUM, MODE_ENUM, stream, true));                       \
+    }                                                                                                       \
+    void cuSZp_decompress_1D_##MODE_NAME##_##TNAME(T* d_decData, unsigned char* d_cmpBytes, size_t nbEle,   \
+                                                    size_t cmpSize, T errorBound, hipStream_t stream) {     \
+        report("cuSZp_decompress_1D_" #MODE_NAME "_" #TNAME,                                                \
+               decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, CUSZP_DIM_1D,      \
+                               kNoDims, TENUM, MODE_ENUM, stream, true));                                   \
+    }
+
+#define CUSZP_ENTRY_ND(DIMN, DIM_ENUM, MODE_NAME, MODE_ENUM, T, TNAME, TENUM)                               \
+    void cuSZp_compress_##DIMN##_##MODE_NAME##_##TNAME(T* d_oriData, unsigned char* d_cmpBytes,             \
+                                                        size_t nbEle, size_t* cmpSize, cuszp_dims_t dims,   \
+                                                        T errorBound, hipStream_t stream) {                 \
+        report("cuSZp_compress_" #DIMN "_" #MODE_NAME "_" #TNAME,                                           \
+               compress_impl(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr, (double)errorBound, \
+                             DIM_ENUM, dims, TENUM, MODE_ENUM, stream, true));                              \
+    }                                                                                                       \
+    void cuSZp_decompress_##DIMN##_##MODE_NAME##_##TNAME(T* d_decData, unsigned char* d_cmpBytes,           \
+                                                          size_t nbEle, size_t cmpSize, cuszp_dims_t dims,  \
+                                                          T errorBound, hipStream_t stream) {               \
+        report("cuSZp_decompress_" #DIMN "_" #MODE_NAME "_" #TNAME,                                         \
+               decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, DIM_ENUM, dims,    \
+                               TENUM, MODE_ENUM, stream, true));                                            \
+    }
+
+#define CUSZP_ENTRIES_T(T, TNAME, TENUM)                                                   \
+    CUSZP_ENTRY_1D(fixed, CUSZP_MODE_FIXED, T, TNAME, TENUM)                              \
+    CUSZP_ENTRY_1D(plain, CUSZP_MODE_PLAIN, T, TNAME, TENUM)                              \
+    CUSZP_ENTRY_1D(outlier, CUSZP_MODE_OUTLIER, T, TNAME, TENUM)                          \
+    CUSZP_ENTRY_ND(2D, CUSZP_DIM_2D, fixed, CUSZP_MODE_FIXED, T, TNAME, TENUM)            \
+    CUSZP_ENTRY_ND(2D, CUSZP_DIM_2D, plain, CUSZP_MODE_PLAIN, T, TNAME, TENUM)            \
+    CUSZP_ENTRY_ND(2D, CUSZP_DIM_2D, outlier, CUSZP_MODE_OUTLIER, T, TNAME, TENUM)        \
+    CUSZP_ENTRY_ND(3D, CUSZP_DIM_3D, fixed, CUSZP_MODE_FIXED, T, TNAME, TENUM)            \
+    CUSZP_ENTRY_ND(3D, CUSZP_DIM_3D, plain, CUSZP_MODE_PLAIN, T, TNAME, TENUM)            \
+    CUSZP_ENTRY_ND(3D, CUSZP_DIM_3D, outlier, CUSZP_MODE_OUTLIER, T, TNAME, TENUM)
+
+CUSZP_ENTRIES_T(float, f32, CUSZP_TYPE_FLOAT)
+CUSZP_ENTRIES_T(double, f64, CUSZP_TYPE_DOUBLE)
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""cuSZp's Python binding (python/pycuSZp.py of the reference), re-pointed at
+libcuszp_amd.so and PyTorch-ROCm tensors.
+
+`cuSZp.compress` / `cuSZp.decompress` keep the reference's argument names,
+order, defaults and return values (python/pycuSZp.py:38-77). Pointers may be
+given as integers (e.g. `tensor.data_ptr()`), ctypes pointers, or torch tensors;
+`stream` may be None (null stream), an integer handle or a torch.cuda.Stream.
+
+On top of that, `compress_tensor` / `decompress_tensor` allocate outputs on the
+tensor's device and return them, and `rel_to_abs` turns a relative bound into
+the absolute one exactly as the reference CLI does (examples/cuSZp.cpp:176-210).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+from ._lib import Uint3, load
+
+lib = load()
+
+CUSZP_DIM_1D, CUSZP_DIM_2D, CUSZP_DIM_3D = 1, 2, 3
+CUSZP_MODE_FIXED, CUSZP_MODE_PLAIN, CUSZP_MODE_OUTLIER = 0, 1, 2
+CUSZP_TYPE_FLOAT, CUSZP_TYPE_DOUBLE = 0, 1
+MODES = {"fixed": 0, "plain": 1, "outlier": 2}
+
+
+class CuSZpError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = lib.cuSZp_status_string(status).decode()
+        super().__init__(f"{what}: {msg}" if what else msg)
+
+
+def _ptr(p):
+    if p is None:
+        return None
+    if hasattr(p, "data_ptr"):
+        return ctypes.c_void_p(p.data_ptr())
+    if isinstance(p, ctypes.c_void_p):
+        return p
+    return ctypes.c_void_p(int(p))
+
+
+def _stream(s):
+    if s is None:
+        return ctypes.c_void_p(0)
+    if hasattr(s, "cuda_stream"):
+        return ctypes.c_void_p(s.cuda_stream)
+    if isinstance(s, ctypes.c_void_p):
+        return s
+    return ctypes.c_void_p(int(s))
+
+
+class cuSZp:
+    """Same surface as the reference's `cuSZp` class (python/pycuSZp.py:38)."""
+
+    def __init__(self):
+        pass
+
+    def compress(self, d_oriData, d_cmpBytes, num_elements, error_bound=0.01,
+                 dim=1, dims=(0, 0, 0), data_type=0, mode=0, stream=None):
+        compressed_size = ctypes.c_size_t(0)
+        dims_struct = Uint3(dims[0], dims[1], dims[2])
+        lib.cuSZp_compress(
+            _ptr(d_oriData),
+            _ptr(d_cmpBytes),
+            num_elements,
+            ctypes.byref(compressed_size),
+            ctypes.c_float(error_bound),
+            ctypes.c_int(dim),
+            dims_struct,
+            ctypes.c_int(data_type),
+            ctypes.c_int(mode),
+            _stream(stream),
+        )
+        return compressed_size.value
+
+    def decompress(self, d_decData, d_cmpBytes, num_elements, compressed_size,
+                   error_bound=0.01, dim=1, dims=(0, 0, 0), data_type=0, mode=0, stream=None):
+        dims_struct = Uint3(dims[0], dims[1], dims[2])
+        lib.cuSZp_decompress(
+            _ptr(d_decData),
+            _ptr(d_cmpBytes),
+            num_elements,
+            compressed_size,
+            ctypes.c_float(error_bound),
+            ctypes.c_int(dim),
+            dims_struct,
+            ctypes.c_int(data_type),
+            ctypes.c_int(mode),
+            _stream(stream),
+        )
+
+
+# ---------------------------------------------------------------------------
+# status-checked helpers
+# ---------------------------------------------------------------------------
+def max_compressed_size(num_elements: int, dim: int = 1, dims=(0, 0, 0), data_type: int = 0, mode: int = 0) -> int:
+    return lib.cuSZp_max_compressed_size(num_elements, dim, Uint3(*dims), data_type, mode)
+
+
+def compress_ex(d_oriData, d_cmpBytes, capacity: int, num_elements: int, error_bound: float,
+                dim: int = 1, dims=(0, 0, 0), data_type: int = 0, mode: int = 0, stream=None) -> int:
+    size = ctypes.c_size_t(0)
+    rc = lib.cuSZp_compress_ex(_ptr(d_oriData), _ptr(d_cmpBytes), capacity, num_elements, ctypes.byref(size),
+                               float(error_bound), dim, Uint3(*dims), data_type, mode, _stream(stream))
+    if rc:
+        raise CuSZpError(rc, "compress")
+    return size.value
+
+
+def decompress_ex(d_decData, d_cmpBytes, num_elements: int, compressed_size: int, error_bound: float,
+                  dim: int = 1, dims=(0, 0, 0), data_type: int = 0, mode: int = 0, stream=None) -> None:
+    rc = lib.cuSZp_decompress_ex(_ptr(d_decData), _ptr(d_cmpBytes), num_elements, compressed_size,
+                                 float(error_bound), dim, Uint3(*dims), data_type, mode, _stream(stream))
+    if rc:
+        raise CuSZpError(rc, "decompress")
+
+
+def _type_of(t) -> int:
+    import torch
+
+    if t.dtype == torch.float32:
+        return CUSZP_TYPE_FLOAT
+    if t.dtype == torch.float64:
+        return CUSZP_TYPE_DOUBLE
+    raise TypeError(f"cuSZp compresses float32/float64 tensors, got {t.dtype}")
+
+
+def rel_to_abs(t, rel_eb: float) -> float:
+    """Relative -> absolute error bound like examples/cuSZp.cpp:176-210:
+    f32: (float)rel * (float)(max - min) in float; f64: rel * (float)(max - min)."""
+    import numpy as np
+
+    mx = t.max().item()
+    mn = t.min().item()
+    if t.dtype.itemsize == 4:
+        return float(np.float32(rel_eb) * (np.float32(mx) - np.float32(mn)))
+    return float(np.float32(rel_eb) * np.float32(mx - mn))
+
+
+def compress_tensor(t, error_bound: float, mode="plain", dim: int = 1, dims: Optional[Tuple[int, int, int]] = None,
+                    stream=None):
+    """Compress a contiguous float32/float64 CUDA tensor. Returns (uint8 tensor, size)."""
+    import torch
+
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    m = MODES[mode] if isinstance(mode, str) else int(mode)
+    typ = _type_of(t)
+    dims = tuple(dims) if dims is not None else (0, 0, 0)
+    cap = max_compressed_size(t.numel(), dim, dims, typ, m)
+    out = torch.empty(max(cap, 1), dtype=torch.uint8, device=t.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(t.device)
+    n = compress_ex(t, out, cap, t.numel(), error_bound, dim, dims, typ, m, stream)
+    return out, n
+
+
+def decompress_tensor(cmp, size: int, num_elements: int, dtype, error_bound: float, mode="plain", dim: int = 1,
+                      dims: Optional[Tuple[int, int, int]] = None, stream=None):
+    import torch
+
+    m = MODES[mode] if isinstance(mode, str) else int(mode)
+    out = torch.empty(num_elements, dtype=dtype, device=cmp.device)
+    typ = _type_of(out)
+    dims = tuple(dims) if dims is not None else (0, 0, 0)
+    if stream is None:
+        stream = torch.cuda.current_stream(cmp.device)
+    decompress_ex(out, cmp, num_elements, size, error_bound, dim, dims, typ, m, stream)
+    return out

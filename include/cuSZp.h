@@ -1,0 +1,183 @@
+/*
+ * cuSZp-compatible C ABI of the MI355X (gfx950) fused error-bounded compressor.
+ *
+ * Drop-in for the reference's public interface:
+ *   - generic entries        include/cuSZp.h:38-45 (reference)   -> cuSZp_compress / cuSZp_decompress
+ *   - per-dimension entries  include/cuSZp/cuSZp_entry_{1,2,3}D_f{32,64}.h:6-11 (reference)
+ *   - enums                  include/cuSZp.h:20-36 (reference), identical values
+ *
+ * Every symbol below is exported with C linkage from libcuszp_amd.so. Device
+ * pointers are plain HIP device pointers (hipMalloc / PyTorch-ROCm tensors);
+ * `stream` is a hipStream_t (0 = null stream).
+ *
+ * Differences from the reference, all deliberate and documented in DESIGN.md:
+ *   - dims is passed as the POD `cuszp_dims_t` {x, y, z} (x fastest), which has
+ *     the layout of CUDA's uint3 and of pycuSZp's ctypes Uint3;
+ *   - *cmpSize is the true stream length (the reference under-reports it for
+ *     single-threadblock grids and for 2D/3D block counts not divisible by 4);
+ *   - decompression writes +0.0 into rate-0 blocks instead of leaving them
+ *     untouched, i.e. it equals the reference's output into a zeroed buffer;
+ *   - invalid arguments (bad enum, 2D/3D dims.x % 4 != 0, nbEle != x*y*z,
+ *     errorBound <= 0 or not finite) print a message and return without
+ *     launching; the *_ex entries return a status code instead.
+ */
+#ifndef CUSZP_AMD_CUSZP_H
+#define CUSZP_AMD_CUSZP_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+#define CUSZP_DEFAULT(v) = v
+extern "C" {
+#else
+#define CUSZP_DEFAULT(v)
+#endif
+
+/* include/cuSZp.h:20-36 (reference) */
+typedef enum {
+    CUSZP_DIM_1D = 1,
+    CUSZP_DIM_2D = 2,
+    CUSZP_DIM_3D = 3
+} cuszp_dim_t;
+
+typedef enum {
+    CUSZP_MODE_FIXED = 0,
+    CUSZP_MODE_PLAIN = 1,
+    CUSZP_MODE_OUTLIER = 2
+} cuszp_mode_t;
+
+typedef enum {
+    CUSZP_TYPE_FLOAT = 0,
+    CUSZP_TYPE_DOUBLE = 1
+} cuszp_type_t;
+
+/* Layout-compatible with CUDA's uint3 and python/pycuSZp.py:8-9 (reference). */
+typedef struct {
+    unsigned int x, y, z;
+} cuszp_dims_t;
+
+typedef enum {
+    CUSZP_SUCCESS = 0,
+    CUSZP_ERROR_INVALID_ARGUMENT = 1,   /* bad enum, null pointer, bad error bound */
+    CUSZP_ERROR_INVALID_DIMS = 2,       /* 2D/3D: x % 4 != 0, zero extent, nbEle != x*y*z */
+    CUSZP_ERROR_TOO_LARGE = 3,          /* stream or block count beyond the format's limits */
+    CUSZP_ERROR_OUTPUT_TOO_SMALL = 4,   /* compressed stream exceeds cmpCapacity */
+    CUSZP_ERROR_CORRUPT_STREAM = 5,     /* cmpSize shorter than the header describes */
+    CUSZP_ERROR_HIP = 6                 /* a HIP runtime call failed */
+} cuszp_status_t;
+
+/* ---- generic entries: include/cuSZp.h:38-45 (reference) ---------------- */
+void cuSZp_compress(void* d_oriData, unsigned char* d_cmpBytes,
+                    size_t nbEle, size_t* cmpSize, float errorBound,
+                    cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
+                    hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress(void* d_decData, unsigned char* d_cmpBytes,
+                      size_t nbEle, size_t cmpSize, float errorBound,
+                      cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
+                      hipStream_t stream CUSZP_DEFAULT(0));
+
+/* ---- 1D entries: include/cuSZp/cuSZp_entry_1D_f32.h:6-11, _1D_f64.h:6-11 (reference) */
+void cuSZp_compress_1D_fixed_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_1D_fixed_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_1D_plain_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_1D_plain_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_1D_outlier_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_1D_outlier_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_1D_fixed_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_1D_fixed_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_1D_plain_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_1D_plain_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_1D_outlier_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_1D_outlier_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+
+/* ---- 2D entries: include/cuSZp/cuSZp_entry_2D_f32.h:6-11, _2D_f64.h:6-11 (reference) */
+void cuSZp_compress_2D_fixed_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_2D_fixed_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_2D_plain_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_2D_plain_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_2D_outlier_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_2D_outlier_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_2D_fixed_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_2D_fixed_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_2D_plain_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_2D_plain_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_2D_outlier_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_2D_outlier_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+
+/* ---- 3D entries: include/cuSZp/cuSZp_entry_3D_f32.h:6-11, _3D_f64.h:6-11 (reference) */
+void cuSZp_compress_3D_fixed_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_3D_fixed_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_3D_plain_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_3D_plain_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_3D_outlier_f32(float* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_3D_outlier_f32(float* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, float errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_3D_fixed_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_3D_fixed_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_3D_plain_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_3D_plain_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_compress_3D_outlier_f64(double* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+void cuSZp_decompress_3D_outlier_f64(double* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, cuszp_dims_t dims, double errorBound, hipStream_t stream CUSZP_DEFAULT(0));
+
+/* ---- extensions (no reference counterpart; the reference returns void and never checks) ---- */
+
+/* Upper bound of the compressed size: header + every block at 32 planes + 4
+ * outlier bytes. The reference sizes its outputs at nbEle*sizeof(T)
+ * (examples/cuSZp_test_f32_1D.cpp:61), which incompressible f32 data overflows. */
+size_t cuSZp_max_compressed_size(size_t nbEle, cuszp_dim_t dim, cuszp_dims_t dims,
+                                 cuszp_type_t type, cuszp_mode_t mode);
+
+/* Status-returning form of cuSZp_compress: validates arguments, never writes
+ * past d_cmpBytes + cmpCapacity, returns once *cmpSize is valid. errorBound is
+ * used as double for f64 and rounded to float for f32. */
+int cuSZp_compress_ex(const void* d_oriData, unsigned char* d_cmpBytes, size_t cmpCapacity,
+                      size_t nbEle, size_t* cmpSize, double errorBound,
+                      cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
+                      hipStream_t stream);
+
+/* Status-returning form of cuSZp_decompress: rejects streams whose header
+ * describes more bytes than cmpSize (those blocks decode as zero). */
+int cuSZp_decompress_ex(void* d_decData, const unsigned char* d_cmpBytes, size_t nbEle,
+                        size_t cmpSize, double errorBound,
+                        cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
+                        hipStream_t stream);
+
+/* Stream-ordered forms: enqueue and return without synchronising. The stream
+ * length is written to the device word *d_cmpSize (compress) when the kernel
+ * finishes; errors are reported by a later cuSZp_last_async_status(stream). */
+int cuSZp_compress_async(const void* d_oriData, unsigned char* d_cmpBytes, size_t cmpCapacity,
+                         size_t nbEle, unsigned long long* d_cmpSize, double errorBound,
+                         cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
+                         hipStream_t stream);
+int cuSZp_decompress_async(void* d_decData, const unsigned char* d_cmpBytes, size_t nbEle,
+                           size_t cmpSize, double errorBound,
+                           cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
+                           hipStream_t stream);
+/* Synchronises `stream` and returns the error status accumulated by its async calls. */
+int cuSZp_last_async_status(hipStream_t stream);
+
+const char* cuSZp_status_string(int status);
+
+/* Library version string ("cuszp_amd <semver> gfx950"). */
+const char* cuSZp_version(void);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+
+/* C++ convenience: accept HIP's uint3/dim3 where the reference takes uint3. */
+#include <hip/hip_vector_types.h>
+inline cuszp_dims_t cuszp_dims(const uint3& d) { cuszp_dims_t r = {d.x, d.y, d.z}; return r; }
+inline void cuSZp_compress(void* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize,
+                           float errorBound, cuszp_dim_t dim, uint3 dims, cuszp_type_t type,
+                           cuszp_mode_t mode, hipStream_t stream = 0) {
+    cuSZp_compress(d_oriData, d_cmpBytes, nbEle, cmpSize, errorBound, dim, cuszp_dims(dims), type, mode, stream);
+}
+inline void cuSZp_decompress(void* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize,
+                             float errorBound, cuszp_dim_t dim, uint3 dims, cuszp_type_t type,
+                             cuszp_mode_t mode, hipStream_t stream = 0) {
+    cuSZp_decompress(d_decData, d_cmpBytes, nbEle, cmpSize, errorBound, dim, cuszp_dims(dims), type, mode, stream);
+}
+#endif
+
+#undef CUSZP_DEFAULT
+#endif /* CUSZP_AMD_CUSZP_H */

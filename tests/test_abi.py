@@ -1,0 +1,104 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol that
+include/cuSZp.h declares, keeps the reference enum values, and rejects bad
+arguments before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from cuszp_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "cuSZp.h")).read()
+    c_part = src.split("#ifdef __cplusplus\n}")[0]
+    return sorted(set(re.findall(r"\b(cuSZp_\w+)\s*\(", c_part)))
+
+
+def test_header_declares_reference_entries():
+    names = _declared()
+    assert "cuSZp_compress" in names and "cuSZp_decompress" in names
+    per_dim = [n for n in names if re.match(r"cuSZp_(de)?compress_[123]D_(fixed|plain|outlier)_f(32|64)$", n)]
+    assert len(per_dim) == 36
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(_declared()) == sorted(_lib.EXPORTED)
+
+
+def test_enum_values_match_reference():
+    src = open(os.path.join(ROOT, "include", "cuSZp.h")).read()
+    for name, val in [("CUSZP_DIM_1D", 1), ("CUSZP_DIM_2D", 2), ("CUSZP_DIM_3D", 3), ("CUSZP_MODE_FIXED", 0),
+                      ("CUSZP_MODE_PLAIN", 1), ("CUSZP_MODE_OUTLIER", 2), ("CUSZP_TYPE_FLOAT", 0),
+                      ("CUSZP_TYPE_DOUBLE", 1)]:
+        assert re.search(rf"{name}\s*=\s*{val}\b", src), name
+
+
+def test_dims_struct_layout():
+    assert ctypes.sizeof(_lib.Uint3) == 12
+
+
+@pytest.fixture(scope="module")
+def L():
+    return _lib.load()
+
+
+def test_max_compressed_size(L, oracle):
+    U = _lib.Uint3
+    for n in (1, 31, 1000, 32768, 32769, 10**6):
+        assert L.cuSZp_max_compressed_size(n, 1, U(0, 0, 0), 0, 1) == oracle.max_size(1, n)
+    assert L.cuSZp_max_compressed_size(12 * 10 * 3, 2, U(12, 10, 3), 1, 2) == oracle.max_size(2, 360, (12, 10, 3))
+    assert L.cuSZp_max_compressed_size(8 * 9 * 5, 3, U(8, 9, 5), 0, 0) == oracle.max_size(3, 360, (8, 9, 5))
+    assert L.cuSZp_max_compressed_size(0, 1, U(0, 0, 0), 0, 1) == 0
+    assert L.cuSZp_max_compressed_size(10, 2, U(5, 2, 1), 0, 1) == 0   # x % 4 != 0
+
+
+@pytest.mark.parametrize(
+    "args,status",
+    [
+        (dict(mode=3), 1),
+        (dict(typ=2), 1),
+        (dict(dim=4), 1),
+        (dict(eb=0.0), 1),
+        (dict(eb=-1.0), 1),
+        (dict(eb=float("nan")), 1),
+        (dict(eb=1e-50), 1),          # rounds to 0 in float
+        (dict(dim=2, dims=(5, 2, 1), n=10), 2),
+        (dict(dim=3, dims=(4, 0, 1), n=0), 2),
+        (dict(dim=2, dims=(4, 4, 4), n=60), 2),
+        (dict(cap=10), 4),
+    ],
+)
+def test_compress_ex_rejects(L, args, status):
+    a = dict(n=1000, eb=1e-3, dim=1, dims=(0, 0, 0), typ=0, mode=1, cap=1 << 20)
+    a.update(args)
+    size = ctypes.c_size_t(123)
+    # non-null dummy pointers: validation must fail before any device access
+    rc = L.cuSZp_compress_ex(ctypes.c_void_p(16), ctypes.c_void_p(16), a["cap"], a["n"], ctypes.byref(size),
+                             a["eb"], a["dim"], _lib.Uint3(*a["dims"]), a["typ"], a["mode"], None)
+    assert rc == status
+    assert size.value == 0
+
+
+def test_decompress_ex_rejects_short_stream(L):
+    rc = L.cuSZp_decompress_ex(ctypes.c_void_p(16), ctypes.c_void_p(16), 100_000, 100, 1e-3, 1,
+                               _lib.Uint3(0, 0, 0), 0, 1, None)
+    assert rc == 5   # header alone is 4096 bytes
+
+
+def test_zero_elements_is_a_noop(L):
+    size = ctypes.c_size_t(9)
+    assert L.cuSZp_compress_ex(None, None, 0, 0, ctypes.byref(size), 1e-3, 1, _lib.Uint3(0, 0, 0), 0, 1, None) == 0
+    assert size.value == 0
+
+
+def test_status_strings(L):
+    for s in range(7):
+        assert L.cuSZp_status_string(s)
+    assert b"gfx950" in L.cuSZp_version()
