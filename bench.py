@@ -1,0 +1,339 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused error-bounded compress/decompress path on one MI355X.
+
+A step is one compress + one decompress of the workload array, input resident
+in HBM, launched through the C ABI (cuSZp_compress_async / _decompress_async)
+on torch's current stream. The JSON line reports:
+
+  value            2 * N * sizeof(T) / step time (uncompressed GB/s over both directions)
+  compress/decompress GB/s, compression ratio, max |x - x'| / eb
+  roofline         the dominant (slower) kernel: algorithmic bytes (N*s + C)
+                   / its mean duration, from HIP events on the launch stream
+  cpu_baseline     the C++ oracle on the host cores, on a bounded sample
+
+Workloads (--workload):
+  3d_outlier_f32   (default) 512^3 f32 Gaussian random field, k^-11/3 spectrum,
+                   [-50,50] + 0.1 % spikes, 3D outlier mode, rel eb 1e-4
+  2d_outlier_f32   same volume as 512 independent 512x512 slices (2D)
+  1d_outlier_f32   same volume, 1D
+  1d_plain_f32     16,777,216 f32 sum of 8 sinusoids + noise, 20 % zero runs, plain, rel 1e-3
+  1d_plain_f32_4g  2^30 f32 sinusoids + noise (4 GiB), plain, rel 1e-3
+  2d_fixed_f64     8192x8192 f64 N(0, 0.02), 2D fixed mode, abs eb 1e-3
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "compress/decompress GB/s + %HBM peak; batched vs per-array GB/s, p99 latency"
+
+WORKLOADS = {
+    "3d_outlier_f32": dict(desc="3D outlier f32 512x512x512 GRF k^-11/3 [-50,50] + 0.1% spikes, rel eb 1e-4",
+                           dim=3, mode=2, dtype="f32", dims=(512, 512, 512), rel=1e-4, gen="grf"),
+    "2d_outlier_f32": dict(desc="2D outlier f32 (x,y,z)=(512,512,512) GRF slices, rel eb 1e-4",
+                           dim=2, mode=2, dtype="f32", dims=(512, 512, 512), rel=1e-4, gen="grf"),
+    "1d_outlier_f32": dict(desc="1D outlier f32 134,217,728 GRF elements, rel eb 1e-4",
+                           dim=1, mode=2, dtype="f32", dims=(512, 512, 512), rel=1e-4, gen="grf"),
+    "1d_plain_f32": dict(desc="1D plain f32 16,777,216 sinusoids+noise, 20% zero runs, rel eb 1e-3",
+                         dim=1, mode=1, dtype="f32", n=16_777_216, rel=1e-3, gen="sin", zero_runs=True),
+    "1d_plain_f32_4g": dict(desc="1D plain f32 1,073,741,824 (4 GiB) sinusoids+noise, rel eb 1e-3",
+                            dim=1, mode=1, dtype="f32", n=1 << 30, rel=1e-3, gen="sin", zero_runs=False),
+    "2d_fixed_f64": dict(desc="2D fixed f64 8192x8192 N(0,0.02), abs eb 1e-3",
+                         dim=2, mode=0, dtype="f64", dims=(8192, 8192, 1), abs=1e-3, gen="normal"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# synthetic inputs, generated on the GPU (seeded)
+# ---------------------------------------------------------------------------
+def gen_sinusoid(torch, n, seed, zero_runs, dtype, dev):
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    f = torch.empty(8, dtype=torch.float64).uniform_(1e-6, 1e-3, generator=g)
+    a = torch.empty(8, dtype=torch.float64).uniform_(0.1, 1.0, generator=g)
+    ph = torch.empty(8, dtype=torch.float64).uniform_(0, 2 * math.pi, generator=g)
+    out = torch.empty(n, dtype=dtype, device=dev)
+    gd = torch.Generator(device=dev).manual_seed(seed)
+    chunk = 1 << 26
+    for s in range(0, n, chunk):
+        m = min(chunk, n - s)
+        i = torch.arange(s, s + m, dtype=torch.float64, device=dev)
+        v = torch.zeros(m, dtype=torch.float64, device=dev)
+        for k in range(8):
+            v += a[k].item() * torch.sin(2 * math.pi * f[k].item() * i + ph[k].item())
+        v += torch.randn(m, dtype=torch.float64, device=dev, generator=gd) * 1e-3
+        out[s:s + m] = v.to(dtype)
+        del i, v
+    if zero_runs:
+        rng = torch.Generator(device="cpu").manual_seed(seed + 1)
+        covered = 0
+        mask = torch.zeros(n, dtype=torch.bool, device=dev)
+        target = int(0.2 * n)
+        while covered < target:
+            ln = int(torch.randint(32, 4097, (1,), generator=rng))
+            st = int(torch.randint(0, max(1, n - ln), (1,), generator=rng))
+            mask[st:st + ln] = True
+            covered += ln
+        out[mask] = 0
+    return out
+
+
+def gen_grf(torch, dims, seed, dtype, dev):
+    nx, ny, nz = dims
+    g = torch.Generator(device=dev).manual_seed(seed)
+    w = torch.randn((nz, ny, nx), dtype=torch.float32, device=dev, generator=g)
+    F = torch.fft.rfftn(w)
+    del w
+    kz = torch.fft.fftfreq(nz, device=dev)[:, None, None]
+    ky = torch.fft.fftfreq(ny, device=dev)[None, :, None]
+    kx = torch.fft.rfftfreq(nx, device=dev)[None, None, :]
+    k = torch.sqrt(kz * kz + ky * ky + kx * kx)
+    k[0, 0, 0] = 1.0
+    F *= k.pow(-11.0 / 6.0)
+    F[0, 0, 0] = 0
+    del k
+    v = torch.fft.irfftn(F, s=(nz, ny, nx))
+    del F
+    v = (v - v.min()) / (v.max() - v.min()) * 100.0 - 50.0
+    v = v.reshape(-1)
+    m = int(round(1e-3 * v.numel()))
+    pos = torch.randperm(v.numel(), device=dev, generator=g)[:m]
+    sign = torch.where(torch.rand(m, device=dev, generator=g) < 0.5, -1.0, 1.0)
+    v[pos] = sign * (100 + 900 * torch.rand(m, device=dev, generator=g))
+    return v.to(dtype).contiguous()
+
+
+def rel_to_abs(torch, x, rel):
+    """examples/cuSZp.cpp:176-210 of the reference: eb = rel * (max - min), in float."""
+    import numpy as np
+
+    mx, mn = x.max().item(), x.min().item()
+    if x.dtype == torch.float32:
+        return float(np.float32(rel) * (np.float32(mx) - np.float32(mn)))
+    return float(np.float32(rel) * np.float32(mx - mn))
+
+
+# ---------------------------------------------------------------------------
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="3d_outlier_f32", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-sample-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the stream shards with an 8-byte exchange per GPU, see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cuszp_amd import _lib
+
+    L = _lib.load()
+    wl = WORKLOADS[args.workload]
+    dev = torch.device("cuda:0")
+    tdtype = torch.float32 if wl["dtype"] == "f32" else torch.float64
+    s = 4 if wl["dtype"] == "f32" else 8
+    typ = 0 if wl["dtype"] == "f32" else 1
+
+    t0 = time.time()
+    if wl["gen"] == "grf":
+        x = gen_grf(torch, wl["dims"], 1, tdtype, dev)
+    elif wl["gen"] == "sin":
+        x = gen_sinusoid(torch, wl["n"], 0 if wl.get("zero_runs") else 2, wl.get("zero_runs", False), tdtype, dev)
+    else:
+        g = torch.Generator(device=dev).manual_seed(2)
+        nx, ny, nz = wl["dims"]
+        x = (torch.randn(nx * ny * nz, dtype=torch.float64, device=dev, generator=g) * 0.02).to(tdtype)
+    torch.cuda.synchronize()
+    n = x.numel()
+    dims = wl.get("dims", (0, 0, 0)) if wl["dim"] > 1 else (0, 0, 0)
+    eb = wl["abs"] if "abs" in wl else rel_to_abs(torch, x, wl["rel"])
+    log(f"[bench] workload {args.workload}: n={n} eb={eb:.6g} generated in {time.time() - t0:.1f}s")
+
+    U = _lib.Uint3(*dims)
+    cap = L.cuSZp_max_compressed_size(n, wl["dim"], U, typ, wl["mode"])
+    cmp = torch.empty(cap, dtype=torch.uint8, device=dev)
+    y = torch.empty_like(x)
+    d_size = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    xp, cp, yp = ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(cmp.data_ptr()), ctypes.c_void_p(y.data_ptr())
+    dsp = ctypes.c_void_p(d_size.data_ptr())
+
+    # one blocking round trip for the stream length and a correctness check
+    size = ctypes.c_size_t(0)
+    rc = L.cuSZp_compress_ex(xp, cp, cap, n, ctypes.byref(size), eb, wl["dim"], U, typ, wl["mode"], sp)
+    assert rc == 0, L.cuSZp_status_string(rc)
+    C = size.value
+    rc = L.cuSZp_decompress_ex(yp, cp, n, C, eb, wl["dim"], U, typ, wl["mode"], sp)
+    assert rc == 0, L.cuSZp_status_string(rc)
+    ebT = float(np.float32(eb)) if typ == 0 else eb
+    fin = torch.isfinite(x) & (x.abs() * (0.5 / ebT) < 2.0**30)
+    max_err = ((x.double() - y.double()).abs()[fin].max() / ebT).item()
+    log(f"[bench] C={C} CR={n * s / C:.3f} max_err/eb={max_err:.7f}")
+
+    def step(evs=None):
+        if evs is not None:
+            evs[0].record(stream)
+        rc1 = L.cuSZp_compress_async(xp, cp, cap, n, dsp, eb, wl["dim"], U, typ, wl["mode"], sp)
+        if evs is not None:
+            evs[1].record(stream)
+        rc2 = L.cuSZp_decompress_async(yp, cp, n, C, eb, wl["dim"], U, typ, wl["mode"], sp)
+        if evs is not None:
+            evs[2].record(stream)
+        if rc1 or rc2:
+            raise RuntimeError(f"launch failed: {rc1} {rc2}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    assert L.cuSZp_last_async_status(sp) == 0
+    assert int(d_size.item()) == C, "stream length changed between runs"
+
+    tc = [e[0].elapsed_time(e[1]) * 1e-3 for e in evs]
+    td = [e[1].elapsed_time(e[2]) * 1e-3 for e in evs]
+    tc_mean, td_mean = sum(tc) / len(tc), sum(td) / len(td)
+    ms_per_step = wall / args.steps * 1e3
+    raw = n * s
+    value = 2 * raw / (wall / args.steps) / 1e9
+    algo = raw + C                       # algorithmic HBM bytes of either kernel
+    comp_gbps = raw / tc_mean / 1e9
+    decomp_gbps = raw / td_mean / 1e9
+    dom = "compress" if tc_mean >= td_mean else "decompress"
+    t_dom = max(tc_mean, td_mean)
+    achieved = algo / t_dom / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(dom)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(np, x, wl, eb, s, args.cpu_sample_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": wl["dtype"],
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {
+            "workload": wl["desc"],
+            "name": args.workload,
+            "elements": n,
+            "dims": list(dims) if wl["dim"] > 1 else None,
+            "dim": wl["dim"],
+            "mode": ["fixed", "plain", "outlier"][wl["mode"]],
+            "error_bound": eb,
+            "compressed_bytes": C,
+            "compression_ratio": round(raw / C, 4),
+            "compress_GBps": round(comp_gbps, 2),
+            "decompress_GBps": round(decomp_gbps, 2),
+            "compress_ms": round(tc_mean * 1e3, 4),
+            "decompress_ms": round(td_mean * 1e3, 4),
+            "compress_hbm_frac": round((raw + C) / tc_mean / 1e9 / HBM_PEAK_GBPS, 4),
+            "decompress_hbm_frac": round((raw + C) / td_mean / 1e9 / HBM_PEAK_GBPS, 4),
+            "max_err_over_eb": round(max_err, 8),
+        },
+        "roofline": {
+            "kernel": f"{dom}_kernel",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": algo,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(np, x, wl, eb, s, budget_s):
+    """Oracle (C++/OpenMP, all host threads) on a leading slice of the same array."""
+    from oracle import oracle as O
+
+    threads = O.max_threads()
+    n = x.numel()
+    if wl["dim"] == 3:
+        nx, ny, nz = wl["dims"]
+        zs = min(nz, 64)
+        dims, m = (nx, ny, zs), nx * ny * zs
+        sample = f"first {zs} z-planes ({m} elements) of the volume"
+    elif wl["dim"] == 2:
+        nx, ny, nz = wl["dims"]
+        if nz > 1:
+            zs = min(nz, 64)
+            dims, m = (nx, ny, zs), nx * ny * zs
+            sample = f"first {zs} slices ({m} elements)"
+        else:
+            ys = min(ny, max(8, (16 << 20) // nx // 8 * 8))
+            dims, m = (nx, ys, 1), nx * ys
+            sample = f"first {ys} rows ({m} elements)"
+    else:
+        m = min(n, 16 << 20)
+        dims = (0, 0, 0)
+        sample = f"first {m} elements"
+    h = x[:m].cpu().numpy()
+    tc = td = 0.0
+    reps = 0
+    t_all = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        c = O.compress(h, eb, wl["mode"], wl["dim"], dims, nthreads=threads)
+        t1 = time.perf_counter()
+        O.decompress(c, m, h.dtype, eb, wl["mode"], wl["dim"], dims, nthreads=threads)
+        t2 = time.perf_counter()
+        tc += t1 - t0
+        td += t2 - t1
+        reps += 1
+        if time.perf_counter() - t_all > budget_s or reps >= 50:
+            break
+    raw = m * s
+    return {
+        "value": round(2 * raw * reps / (tc + td) / 1e9, 4),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample}, {reps} compress+decompress round trips",
+        "compress_GBps": round(raw * reps / tc / 1e9, 4),
+        "decompress_GBps": round(raw * reps / td / 1e9, 4),
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -34,7 +34,7 @@ struct Slot {
     unsigned long long* d_status = nullptr;   // tile status words
     uint64_t cap_tiles = 0;
     unsigned long long* d_misc = nullptr;     // [0] ticket, [1] total, [2] error bits
-    unsigned long long* h_res = nullptr;      // pinned [2]
+    unsigned long long* h_res = nullptr;      // pinned copy of d_misc[1..3)
     uint32_t epoch = 0;
 };
 
@@ -82,13 +82,35 @@ int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
     return CUSZP_SUCCESS;
 }
 
-ScanArgs scan_args(const Slot& s) {
+int status_from_bits(unsigned long long bits) {
+    if (bits & kErrOverflow) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
+    if (bits & kErrTruncated) return CUSZP_ERROR_CORRUPT_STREAM;
+    return CUSZP_SUCCESS;
+}
+
+// d_misc: [0] ticket counter (self-resetting), [1] total of blocking calls,
+// [2] sticky error bits (low 32 bits).
+ScanArgs scan_args(const Slot& s, unsigned long long* total) {
     ScanArgs a;
     a.status = s.d_status;
     a.ticket = s.d_misc;
-    a.result = s.d_misc + 1;
+    a.total = total ? total : s.d_misc + 1;
+    a.err = (unsigned int*)(s.d_misc + 2);
     a.tag = (unsigned long long)s.epoch << kEpochShift;
     return a;
+}
+
+// Read {total, error bits} after a blocking launch; clear errors once seen.
+int finish_blocking(Slot& s, hipStream_t st, size_t* total) {
+    CUSZP_HIP(hipMemcpyAsync(s.h_res, s.d_misc + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    CUSZP_HIP(hipStreamSynchronize(st));
+    const unsigned long long bits = s.h_res[1] & 0xffffffffull;
+    if (total) *total = (size_t)s.h_res[0];
+    if (bits) {
+        CUSZP_HIP(hipMemsetAsync(s.d_misc + 2, 0, sizeof(unsigned long long), st));
+        CUSZP_HIP(hipStreamSynchronize(st));
+    }
+    return status_from_bits(bits);
 }
 
 // Geometry of a (dim, nbEle, dims) triple; mirrors the reference's block counts.
@@ -139,13 +161,6 @@ bool bad_eb(double eb, int type) {
     return !(eb > 0.0) || !std::isfinite(eb);
 }
 
-// Async error bits accumulated per slot for cuSZp_last_async_status.
-int status_from_bits(unsigned long long bits) {
-    if (bits & kErrOverflow) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
-    if (bits & kErrTruncated) return CUSZP_ERROR_CORRUPT_STREAM;
-    return CUSZP_SUCCESS;
-}
-
 int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
                   unsigned long long* d_size, double eb, int dim, cuszp_dims_t dims, int type, int mode,
                   hipStream_t st, bool blocking) {
@@ -167,21 +182,15 @@ int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size
     std::lock_guard<std::mutex> lk(s.mu);
     rc = prepare(s, g.ntiles, st);
     if (rc) return rc;
-    CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, (blocking ? 3 : 2) * sizeof(unsigned long long), st));
-    const ScanArgs sa = scan_args(s);
+    const ScanArgs sa = scan_args(s, d_size);
     hipError_t e;
     if (type == CUSZP_TYPE_FLOAT)
         e = launch_compress_f32(dim, mode, (const float*)in, out, g, (float)eb, sa, cap, st);
     else
         e = launch_compress_f64(dim, mode, (const double*)in, out, g, eb, sa, cap, st);
     if (e != hipSuccess) return CUSZP_ERROR_HIP;
-    if (d_size) CUSZP_HIP(hipMemcpyAsync(d_size, s.d_misc + 1, sizeof(*d_size), hipMemcpyDeviceToDevice, st));
     if (!blocking) return CUSZP_SUCCESS;
-    CUSZP_HIP(hipMemcpyAsync(s.h_res, s.d_misc + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    CUSZP_HIP(hipStreamSynchronize(st));
-    rc = status_from_bits(s.h_res[1]);
-    if (cmp_size) *cmp_size = (size_t)s.h_res[0];
-    return rc;
+    return finish_blocking(s, st, cmp_size);
 }
 
 int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_size, double eb, int dim,
@@ -200,8 +209,7 @@ int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
     std::lock_guard<std::mutex> lk(s.mu);
     rc = prepare(s, g.ntiles, st);
     if (rc) return rc;
-    CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, (blocking ? 3 : 2) * sizeof(unsigned long long), st));
-    const ScanArgs sa = scan_args(s);
+    const ScanArgs sa = scan_args(s, nullptr);
     hipError_t e;
     if (type == CUSZP_TYPE_FLOAT)
         e = launch_decompress_f32(dim, mode, (float*)out, cmp, g, (float)eb, sa, cmp_size, st);
@@ -209,9 +217,7 @@ int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
         e = launch_decompress_f64(dim, mode, (double*)out, cmp, g, eb, sa, cmp_size, st);
     if (e != hipSuccess) return CUSZP_ERROR_HIP;
     if (!blocking) return CUSZP_SUCCESS;
-    CUSZP_HIP(hipMemcpyAsync(s.h_res, s.d_misc + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    CUSZP_HIP(hipStreamSynchronize(st));
-    return status_from_bits(s.h_res[1]);
+    return finish_blocking(s, st, nullptr);
 }
 
 void report(const char* fn, int rc) {
@@ -285,13 +291,7 @@ int cuSZp_last_async_status(hipStream_t stream) {
     Slot& s = slot_for(dev, stream);
     std::lock_guard<std::mutex> lk(s.mu);
     if (!s.d_misc) return CUSZP_SUCCESS;
-    if (hipMemcpyAsync(s.h_res, s.d_misc + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess)
-        return CUSZP_ERROR_HIP;
-    const int rc = status_from_bits(s.h_res[1]);
-    if (hipMemsetAsync(s.d_misc + 2, 0, sizeof(unsigned long long), stream) != hipSuccess) return CUSZP_ERROR_HIP;
-    return rc;
+    return finish_blocking(s, stream, nullptr);
 }
 
 // ---- generic drop-ins: src/cuSZp.cu:4-83, :86-165 ------------------------

@@ -41,7 +41,7 @@ constexpr uint64_t kFlagPrefix = 2ull << 38;
 constexpr uint64_t kFlagMask = 3ull << 38;
 constexpr uint64_t kValueMask = (1ull << 38) - 1;
 
-// Error bits reported through ScanArgs::result[1].
+// Error bits reported through ScanArgs::err.
 constexpr uint32_t kErrOverflow = 1u;    // compressed stream exceeds output capacity
 constexpr uint32_t kErrTruncated = 2u;   // stream shorter than its header describes
 
@@ -57,8 +57,9 @@ struct Geom {
 
 struct ScanArgs {
     unsigned long long* status;     // [ntiles] tile status words
-    unsigned long long* ticket;     // tile ticket counter, zeroed in-stream before each launch
-    unsigned long long* result;     // [0] total stream bytes, [1] error bits
+    unsigned long long* ticket;     // tile ticket counter; 0 at launch, reset to 0 by the last claim
+    unsigned long long* total;      // out: total stream bytes (written by the last tile), may be null
+    unsigned int* err;              // sticky error bits (kErr*)
     unsigned long long tag;         // epoch << kEpochShift
 };
 
@@ -318,10 +319,14 @@ __device__ __forceinline__ TileOffsets tile_scan(const ScanArgs& sa, uint64_t ti
 }
 
 // Tiles are numbered in the order workgroups start, so a tile only ever waits
-// on tiles whose workgroups are already resident.
-__device__ __forceinline__ uint64_t claim_tile(const ScanArgs& sa, uint64_t* s_tile) {
+// on tiles whose workgroups are already resident. The workgroup that draws the
+// last ticket returns the counter to 0 for the next launch on this scan state:
+// every other draw of this launch precedes its own in the counter's order.
+__device__ __forceinline__ uint64_t claim_tile(const ScanArgs& sa, uint64_t ntiles, uint64_t* s_tile) {
     if (threadIdx.x == 0) {
-        *s_tile = __hip_atomic_fetch_add(sa.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t = __hip_atomic_fetch_add(sa.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == ntiles - 1) __hip_atomic_store(sa.ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *s_tile = t;
     }
     __syncthreads();
     return *s_tile;
@@ -452,8 +457,8 @@ compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom 
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint64_t tile = claim_tile(sa, &s_tile);
-    if (tile >= geo.ntiles) return;   // unreachable: the counter is reset before every launch
+    const uint64_t tile = claim_tile(sa, geo.ntiles, &s_tile);
+    if (tile >= geo.ntiles) return;   // unreachable: exactly ntiles draws per launch
     const uint64_t grp0 = (tile * kWaves + (uint64_t)wave) * G;
 
     LaneGeo<DIM> lg;
@@ -562,11 +567,11 @@ compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom 
     }
 
     const TileOffsets to = tile_scan(sa, tile, wtot, wave, lane, s_wtot, &s_excl);
-    if (tile == geo.ntiles - 1 && threadIdx.x == 0) sa.result[0] = geo.H + to.tile_incl;
+    if (tile == geo.ntiles - 1 && threadIdx.x == 0 && sa.total) *sa.total = geo.H + to.tile_incl;
 
     uint64_t base = geo.H + to.wave_base;
     if (base + wtot > cap) {   // never write past the caller's buffer
-        if (lane == 0) atomicOr((unsigned int*)(sa.result + 1), kErrOverflow);
+        if (lane == 0) atomicOr(sa.err, kErrOverflow);
         return;
     }
 #pragma unroll
@@ -614,8 +619,8 @@ decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Ge
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint64_t tile = claim_tile(sa, &s_tile);
-    if (tile >= geo.ntiles) return;   // unreachable: the counter is reset before every launch
+    const uint64_t tile = claim_tile(sa, geo.ntiles, &s_tile);
+    if (tile >= geo.ntiles) return;   // unreachable: exactly ntiles draws per launch
     const uint64_t grp0 = (tile * kWaves + (uint64_t)wave) * G;
 
     // header bytes and block sizes of this wave's groups: lane g <-> group g
@@ -637,10 +642,10 @@ decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Ge
     const uint32_t pre = incl - sz;
     const uint64_t wtot = rdlane(incl, 63);
     const TileOffsets to = tile_scan(sa, tile, wtot, wave, lane, s_wtot, &s_excl);
-    if (tile == geo.ntiles - 1 && threadIdx.x == 0) sa.result[0] = geo.H + to.tile_incl;
+    if (tile == geo.ntiles - 1 && threadIdx.x == 0 && sa.total) *sa.total = geo.H + to.tile_incl;
     const uint64_t base = geo.H + to.wave_base;
     const bool intact = base + wtot <= cmp_size;
-    if (!intact && lane == 0) atomicOr((unsigned int*)(sa.result + 1), kErrTruncated);
+    if (!intact && lane == 0) atomicOr(sa.err, kErrTruncated);
 
     LaneGeo<DIM> lg;
     lg.init(lane, geo);
