@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -33,7 +34,7 @@ struct Slot {
     std::mutex mu;
     unsigned long long* d_status = nullptr;   // tile status words
     uint64_t cap_tiles = 0;
-    unsigned long long* d_misc = nullptr;     // [0] ticket, [1] total, [2] error bits
+    unsigned long long* d_misc = nullptr;     // [1] total, [2] error bits
     unsigned long long* h_res = nullptr;      // pinned copy of d_misc[1..3)
     uint32_t epoch = 0;
 };
@@ -57,9 +58,9 @@ Slot& slot_for(int dev, hipStream_t st) {
 // Make room for `ntiles` status words and advance the epoch. Caller holds s.mu.
 int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
     if (!s.d_misc) {
-        CUSZP_HIP(hipMalloc((void**)&s.d_misc, 4 * sizeof(unsigned long long)));
-        CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, 4 * sizeof(unsigned long long), st));
-        CUSZP_HIP(hipHostMalloc((void**)&s.h_res, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+        CUSZP_HIP(hipMalloc((void**)&s.d_misc, 8 * sizeof(unsigned long long)));
+        CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, 8 * sizeof(unsigned long long), st));
+        CUSZP_HIP(hipHostMalloc((void**)&s.h_res, 8 * sizeof(unsigned long long), hipHostMallocDefault));
     }
     if (ntiles > s.cap_tiles) {
         if (s.d_status) {
@@ -83,19 +84,20 @@ int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
 }
 
 int status_from_bits(unsigned long long bits) {
+    if (bits & kErrStall) return CUSZP_ERROR_INTERNAL;
     if (bits & kErrOverflow) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
     if (bits & kErrTruncated) return CUSZP_ERROR_CORRUPT_STREAM;
     return CUSZP_SUCCESS;
 }
 
-// d_misc: [0] ticket counter (self-resetting), [1] total of blocking calls,
-// [2] sticky error bits (low 32 bits).
+// d_misc: [1] total of blocking calls, [2] sticky error bits (low 32 bits),
+// [4..6] watchdog diagnostics.
 ScanArgs scan_args(const Slot& s, unsigned long long* total) {
     ScanArgs a;
     a.status = s.d_status;
-    a.ticket = s.d_misc;
     a.total = total ? total : s.d_misc + 1;
     a.err = (unsigned int*)(s.d_misc + 2);
+    a.diag = s.d_misc + 4;
     a.tag = (unsigned long long)s.epoch << kEpochShift;
     return a;
 }
@@ -106,8 +108,13 @@ int finish_blocking(Slot& s, hipStream_t st, size_t* total) {
     CUSZP_HIP(hipStreamSynchronize(st));
     const unsigned long long bits = s.h_res[1] & 0xffffffffull;
     if (total) *total = (size_t)s.h_res[0];
+    if (bits & kErrStall) {
+        CUSZP_HIP(hipMemcpy(s.h_res + 4, s.d_misc + 4, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        std::fprintf(stderr, "cuSZp: internal error: tile %llu stopped waiting for tile %llu (status word 0x%016llx)\n",
+                     s.h_res[4] - 1, s.h_res[5], s.h_res[6]);
+    }
     if (bits) {
-        CUSZP_HIP(hipMemsetAsync(s.d_misc + 2, 0, sizeof(unsigned long long), st));
+        CUSZP_HIP(hipMemsetAsync(s.d_misc + 2, 0, 6 * sizeof(unsigned long long), st));
         CUSZP_HIP(hipStreamSynchronize(st));
     }
     return status_from_bits(bits);
@@ -120,7 +127,6 @@ int make_geom(int dim, size_t n, cuszp_dims_t d, Geom& g) {
         g.n = n;
         g.nblocks = (n + 31) / 32;
         g.H = (n + 32767) / 32768 * 1024;          // src/cuSZp_entry_1D_f32.cu:43
-        g.ntiles = g.H / 2 / kGroupsPerTile;       // 2 header bytes per group
         return CUSZP_SUCCESS;
     }
     if (dim != CUSZP_DIM_2D && dim != CUSZP_DIM_3D) return CUSZP_ERROR_INVALID_ARGUMENT;
@@ -139,9 +145,32 @@ int make_geom(int dim, size_t n, cuszp_dims_t d, Geom& g) {
     g.nblocks = (uint64_t)g.Bx * g.By * bz;
     if (g.nblocks >= (1ull << 40)) return CUSZP_ERROR_TOO_LARGE;
     g.H = (g.nblocks + 3) / 4 * 4;                  // src/cuSZp_kernels_2D_f32.cu:46
-    g.ntiles = (g.nblocks + kGroupsPerTile - 1) / kGroupsPerTile;
     return CUSZP_SUCCESS;
 }
+
+}  // namespace
+
+int& grid_divisor(int dev) {
+    static int div[64] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                          1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                          1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+    return div[dev & 63];
+}
+
+TilePlan plan_tiles(int dim, const Geom& g) {
+    static const int forced = [] {
+        const char* e = std::getenv("CUSZP_WAVES");
+        const int w = e ? std::atoi(e) : 0;
+        return (w == 4 || w == 8 || w == 16) ? w : 0;
+    }();
+    const uint64_t groups = (dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;   // 64-element units
+    int waves = forced;
+    if (!waves) waves = 4;
+    const uint64_t per_tile = (uint64_t)waves * kGroupsPerWave;
+    return TilePlan{waves, (groups + per_tile - 1) / per_tile};
+}
+
+namespace {
 
 uint64_t max_stream_bytes(int dim, const Geom& g) {
     const uint64_t S = (dim == CUSZP_DIM_1D) ? 4 : 8;
@@ -161,7 +190,7 @@ bool bad_eb(double eb, int type) {
     return !(eb > 0.0) || !std::isfinite(eb);
 }
 
-int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
+int compress_once(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
                   unsigned long long* d_size, double eb, int dim, cuszp_dims_t dims, int type, int mode,
                   hipStream_t st, bool blocking) {
     if (cmp_size) *cmp_size = 0;
@@ -176,6 +205,8 @@ int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size
     if (!in || !out) return CUSZP_ERROR_INVALID_ARGUMENT;
     if (max_stream_bytes(dim, g) > kMaxStream) return CUSZP_ERROR_TOO_LARGE;
     if (cap < g.H) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
+    const TilePlan plan = plan_tiles(dim, g);
+    g.ntiles = plan.ntiles;
     int dev = 0;
     CUSZP_HIP(hipGetDevice(&dev));
     Slot& s = slot_for(dev, st);
@@ -185,15 +216,15 @@ int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size
     const ScanArgs sa = scan_args(s, d_size);
     hipError_t e;
     if (type == CUSZP_TYPE_FLOAT)
-        e = launch_compress_f32(dim, mode, (const float*)in, out, g, (float)eb, sa, cap, st);
+        e = launch_compress_f32(dim, mode, plan.waves, (const float*)in, out, g, (float)eb, sa, cap, st);
     else
-        e = launch_compress_f64(dim, mode, (const double*)in, out, g, eb, sa, cap, st);
+        e = launch_compress_f64(dim, mode, plan.waves, (const double*)in, out, g, eb, sa, cap, st);
     if (e != hipSuccess) return CUSZP_ERROR_HIP;
     if (!blocking) return CUSZP_SUCCESS;
     return finish_blocking(s, st, cmp_size);
 }
 
-int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_size, double eb, int dim,
+int decompress_once(void* out, const unsigned char* cmp, size_t n, size_t cmp_size, double eb, int dim,
                     cuszp_dims_t dims, int type, int mode, hipStream_t st, bool blocking) {
     if (bad_enums(dim, type, mode) || bad_eb(eb, type)) return CUSZP_ERROR_INVALID_ARGUMENT;
     Geom g;
@@ -203,6 +234,8 @@ int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
     if (!out || !cmp) return CUSZP_ERROR_INVALID_ARGUMENT;
     if (max_stream_bytes(dim, g) > kMaxStream) return CUSZP_ERROR_TOO_LARGE;
     if (cmp_size < g.H) return CUSZP_ERROR_CORRUPT_STREAM;
+    const TilePlan plan = plan_tiles(dim, g);
+    g.ntiles = plan.ntiles;
     int dev = 0;
     CUSZP_HIP(hipGetDevice(&dev));
     Slot& s = slot_for(dev, st);
@@ -212,12 +245,45 @@ int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
     const ScanArgs sa = scan_args(s, nullptr);
     hipError_t e;
     if (type == CUSZP_TYPE_FLOAT)
-        e = launch_decompress_f32(dim, mode, (float*)out, cmp, g, (float)eb, sa, cmp_size, st);
+        e = launch_decompress_f32(dim, mode, plan.waves, (float*)out, cmp, g, (float)eb, sa, cmp_size, st);
     else
-        e = launch_decompress_f64(dim, mode, (double*)out, cmp, g, eb, sa, cmp_size, st);
+        e = launch_decompress_f64(dim, mode, plan.waves, (double*)out, cmp, g, eb, sa, cmp_size, st);
     if (e != hipSuccess) return CUSZP_ERROR_HIP;
     if (!blocking) return CUSZP_SUCCESS;
     return finish_blocking(s, st, nullptr);
+}
+
+// A stalled scan means a persistent grid larger than what is resident: halve
+// every persistent grid on this device and run the call again (the inputs are
+// untouched by a stalled launch).
+bool shrink_after_stall(int rc) {
+    if (rc != CUSZP_ERROR_INTERNAL) return false;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    int& d = grid_divisor(dev);
+    if (d >= 64) return false;
+    d *= 2;
+    std::fprintf(stderr, "cuSZp: persistent grid reduced to 1/%d of the occupancy estimate\n", d);
+    return true;
+}
+
+int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
+                  unsigned long long* d_size, double eb, int dim, cuszp_dims_t dims, int type, int mode,
+                  hipStream_t st, bool blocking) {
+    int rc;
+    do {
+        rc = compress_once(in, out, cap, n, cmp_size, d_size, eb, dim, dims, type, mode, st, blocking);
+    } while (blocking && shrink_after_stall(rc));
+    return rc;
+}
+
+int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_size, double eb, int dim,
+                    cuszp_dims_t dims, int type, int mode, hipStream_t st, bool blocking) {
+    int rc;
+    do {
+        rc = decompress_once(out, cmp, n, cmp_size, eb, dim, dims, type, mode, st, blocking);
+    } while (blocking && shrink_after_stall(rc));
+    return rc;
 }
 
 void report(const char* fn, int rc) {
@@ -242,6 +308,7 @@ const char* cuSZp_status_string(int status) {
         case CUSZP_ERROR_OUTPUT_TOO_SMALL: return "compressed output exceeds the buffer capacity";
         case CUSZP_ERROR_CORRUPT_STREAM: return "compressed stream is shorter than its header describes";
         case CUSZP_ERROR_HIP: return "HIP runtime error";
+        case CUSZP_ERROR_INTERNAL: return "internal error: the device-wide scan stalled (output invalid)";
         default: return "unknown status";
     }
 }

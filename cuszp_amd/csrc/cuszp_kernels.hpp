@@ -9,28 +9,37 @@
 //   * a "group" is 64 consecutive elements of cuSZp block order: two 32-element
 //     blocks in 1D, one 8x8 (2D) or 4x4x4 (3D) block otherwise. One wave holds
 //     one group, one element per lane;
-//   * a workgroup is 4 waves; each wave owns G consecutive groups, so a tile is
-//     4*G groups and tiles are claimed in ticket order;
+//   * a workgroup is WAVES waves; each wave owns G consecutive groups, so a
+//     tile is WAVES*G groups. The grid is persistent and cooperatively
+//     launched (every workgroup resident): workgroup w handles tiles w, w+B,
+//     w+2B, ... and prefetches its next tile while it scans and stores the
+//     current one. No ticket atomics: one hot device-scope atomic serialises
+//     at about 12 ns per operation on MI355X;
 //   * lanes hold elements in "mirror" order: lane L holds element L^7, which is
 //     exactly the bit order of cuSZp's sign/plane bytes (element e at byte e/8,
-//     bit 7-e%8). A 64-lane carry-out mask of v_add_co_u32 therefore IS one
-//     bit-plane of two 1D blocks (or of one 2D/3D block) in stream byte order;
+//     bit 7-e%8). Each lane packs its sign into bit 31 above the 31-bit
+//     magnitude; a 32x32 bit-matrix transpose across each 32-lane half (five
+//     ds_swizzle xor stages) then leaves plane p's stream word in lane p and the
+//     sign word in lane 31 -- for every rate at a fixed cost. Decompression runs
+//     the same transpose (it is an involution);
+//   * per-block bookkeeping (rates, header bytes, sizes, offsets) runs once per
+//     wave with one lane per block;
 //   * the inter-tile exclusive scan of compressed bytes is a single-pass
 //     decoupled lookback: one 64-bit status word per tile packs
 //     {epoch, flag, value}, is read 64 predecessors at a time by one wave, and
 //     needs only relaxed agent-scope atomics (no fences: the value travels in
-//     the same word as its flag). Tickets make forward progress independent of
-//     workgroup dispatch order.
+//     the same word as its flag). With all workgroups resident, the lowest
+//     unfinished tile can always complete, so the walk always terminates.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace cuszp {
 
 constexpr int kWaveSize = 64;
-constexpr int kWaves = 4;
-constexpr int kThreads = kWaveSize * kWaves;
 
 enum : int { kFixed = 0, kPlain = 1, kOutlier = 2 };
 
@@ -44,6 +53,12 @@ constexpr uint64_t kValueMask = (1ull << 38) - 1;
 // Error bits reported through ScanArgs::err.
 constexpr uint32_t kErrOverflow = 1u;    // compressed stream exceeds output capacity
 constexpr uint32_t kErrTruncated = 2u;   // stream shorter than its header describes
+constexpr uint32_t kErrStall = 4u;       // a lookback gave up waiting (internal error; output invalid)
+
+// Bound on lookback polls (each polls the status words once and sleeps ~64
+// cycles): far above any legitimate wait, it guarantees that every wave of a
+// launch exits even if a predecessor tile never publishes.
+constexpr uint32_t kSpinLimit = 1u << 18;
 
 struct Geom {
     uint64_t n;         // elements
@@ -57,9 +72,9 @@ struct Geom {
 
 struct ScanArgs {
     unsigned long long* status;     // [ntiles] tile status words
-    unsigned long long* ticket;     // tile ticket counter; 0 at launch, reset to 0 by the last claim
     unsigned long long* total;      // out: total stream bytes (written by the last tile), may be null
     unsigned int* err;              // sticky error bits (kErr*)
+    unsigned long long* diag;       // [0] first stalled tile + 1, [1] its missing predecessor, [2] tag seen
     unsigned long long tag;         // epoch << kEpochShift
 };
 
@@ -81,36 +96,47 @@ constexpr int kDppQuadShr2 = 0x40;    // quad_perm [0,0,0,1]
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
-// old[l] <- v (v and l wave-uniform)
-__device__ __forceinline__ uint32_t wrlane(uint32_t v, uint32_t l, uint32_t old) {
-    asm("v_writelane_b32 %0, %1, m0" : "+v"(old) : "s"(v), "{m0}"(l));
+// old[L] <- v for a compile-time lane L (v wave-uniform)
+template <int L>
+__device__ __forceinline__ uint32_t wrlane(uint32_t v, uint32_t old) {
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(old) : "s"(v), "i"(L));
     return old;
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
-// b <- 2b; returns the 64-lane mask of the bits shifted out (bit 31 of old b).
-__device__ __forceinline__ uint64_t take_top_bit(uint32_t& b) {
-    uint32_t nb;
-    uint64_t m;
-    asm("v_add_co_u32 %0, %1, %2, %2" : "=v"(nb), "=s"(m) : "v"(b));
-    b = nb;
-    return m;
+// ---------------------------------------------------------------------------
+// 32x32 bit transpose within each 32-lane half: out lane j bit i = in lane i bit j.
+// Stage J exchanges lane-index bit log2(J) with data bit log2(J): the lane whose
+// bit is 0 keeps its data bits with that bit 0 and takes its partner's (lane^J)
+// other half shifted into place. Stages 16 and 8 move whole bytes (v_perm),
+// stages 4, 2, 1 rotate and merge (v_alignbit + v_bfi). The partner value comes
+// from ds_swizzle (xor mode), which runs on the LDS pipe, not the VALU.
+// ---------------------------------------------------------------------------
+struct XposeConsts {
+    uint32_t sh1, mk1, sh2, mk2, sh4, mk4, sel8, sel16;
+    __device__ __forceinline__ void init(int lane) {
+        const int l = lane & 31;
+        const bool lo1 = !(l & 1), lo2 = !(l & 2), lo4 = !(l & 4), lo8 = !(l & 8), lo16 = !(l & 16);
+        sh1 = lo1 ? 31 : 1;  mk1 = lo1 ? 0xAAAAAAAAu : 0x55555555u;
+        sh2 = lo2 ? 30 : 2;  mk2 = lo2 ? 0xCCCCCCCCu : 0x33333333u;
+        sh4 = lo4 ? 28 : 4;  mk4 = lo4 ? 0xF0F0F0F0u : 0x0F0F0F0Fu;
+        sel8 = lo8 ? 0x06020400u : 0x03070105u;     // [x0,y0,x2,y2] | [y1,x1,y3,x3]
+        sel16 = lo16 ? 0x05040100u : 0x03020706u;   // [x0,x1,y0,y1] | [y2,y3,x2,x3]
+    }
+};
+template <int J>
+__device__ __forceinline__ uint32_t swizzle_xor(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1f | (J << 10));
 }
-// a <- 2a + (bit `lane` of m)
-__device__ __forceinline__ uint32_t push_bit(uint32_t a, uint64_t m) {
-    uint32_t r;
-    uint64_t c;
-    asm("v_addc_co_u32 %0, %1, %2, %2, %3" : "=v"(r), "=s"(c) : "v"(a), "s"(m));
-    return r;
-}
-
-// OR over each 16-lane row; every lane of a row receives the row's OR.
-__device__ __forceinline__ uint32_t row_or(uint32_t v) {
-    v |= dpp<kDppQuadSwap1>(v);
-    v |= dpp<kDppQuadSwap2>(v);
-    v |= dpp<kDppHalfMirror>(v);
-    v |= dpp<kDppRowMirror>(v);
-    return v;
+__device__ __forceinline__ uint32_t rotr(uint32_t v, uint32_t s) { return __builtin_amdgcn_alignbit(v, v, s); }
+__device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { return (m & a) | (~m & b); }
+__device__ __forceinline__ uint32_t xpose32(uint32_t x, const XposeConsts& c) {
+    x = __builtin_amdgcn_perm(swizzle_xor<16>(x), x, c.sel16);
+    x = __builtin_amdgcn_perm(swizzle_xor<8>(x), x, c.sel8);
+    x = bfi(c.mk4, rotr(swizzle_xor<4>(x), c.sh4), x);
+    x = bfi(c.mk2, rotr(swizzle_xor<2>(x), c.sh2), x);
+    x = bfi(c.mk1, rotr(swizzle_xor<1>(x), c.sh1), x);
+    return x;
 }
 
 // Inclusive prefix sums (wrapping u32).
@@ -132,13 +158,7 @@ __device__ __forceinline__ uint32_t scan_wave(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWaveSize);
-    return v;
-}
-
-__device__ __forceinline__ int bitlen(uint32_t v) { return v ? 32 - __builtin_clz(v) : 0; }
+__device__ __forceinline__ uint32_t bitlen(uint32_t v) { return v ? 32u - (uint32_t)__builtin_clz(v) : 0u; }
 
 // ---------------------------------------------------------------------------
 // quantisation: t = RN(x*r); s = (t >= -0.5 ? 0 : 1); q = sat_trunc(RN(t+0.5)) - s
@@ -191,24 +211,22 @@ __device__ __forceinline__ BlockView view_of(uint32_t byte) {
     return v;
 }
 
-// Header byte for one block given the OR of its ranked magnitudes excluding
-// element 0 (or2), element 0's ranked magnitude (r0) and raw magnitude (o).
+// Header byte of a block from its rate over all elements (fr1), over elements
+// e >= 1 (fr2) and the bit length of element 0's magnitude (outlier mode).
 // src/cuSZp_kernels_1D_f32.cu:1165-1183 (outlier), :641 (fixed/plain).
 template <int MODE, int S>
-__device__ __forceinline__ uint32_t rate_byte(uint32_t or2, uint32_t r0, uint32_t o) {
-    const int fr1 = bitlen(or2 | r0);
+__device__ __forceinline__ uint32_t rate_byte(uint32_t fr1, uint32_t fr2, uint32_t o_bits) {
     if constexpr (MODE != kOutlier) {
-        return (uint32_t)fr1;
+        return fr1;
     } else {
-        const int fr2 = bitlen(or2);
-        const int nb = (bitlen(o) + 7) / 8;
-        const int ofs1 = fr1 ? S + fr1 * S : 0;
-        const int ofs2 = fr2 ? S + fr2 * S + nb : S + nb;
-        return (ofs1 <= ofs2) ? (uint32_t)fr1 : (uint32_t)((fr2 | 0x80 | ((nb - 1) << 5)) & 0xff);
+        const uint32_t nb = (o_bits + 7) / 8;
+        const uint32_t ofs1 = fr1 ? S + fr1 * S : 0;
+        const uint32_t ofs2 = fr2 ? S + fr2 * S + nb : S + nb;
+        return (ofs1 <= ofs2) ? fr1 : ((fr2 | 0x80u | ((nb - 1) << 5)) & 0xffu);
     }
 }
 
-// Byte-granular little-endian store / load of 1..4 bytes (outlier prefixes).
+// Byte-granular little-endian store of 1..4 bytes (outlier prefixes).
 __device__ __forceinline__ void store_le(uint8_t* p, uint32_t v, uint32_t nb) {
     typedef uint32_t __attribute__((aligned(1))) u32u;
     typedef uint16_t __attribute__((aligned(1))) u16u;
@@ -218,11 +236,6 @@ __device__ __forceinline__ void store_le(uint8_t* p, uint32_t v, uint32_t nb) {
         if (nb & 2) { *(u16u*)p = (uint16_t)v; p += 2; v >>= 16; }
         if (nb & 1) { *p = (uint8_t)v; }
     }
-}
-__device__ __forceinline__ uint32_t load_le(const uint8_t* p, uint32_t nb) {
-    uint32_t v = 0;
-    for (uint32_t k = 0; k < nb; ++k) v |= (uint32_t)p[k] << (8 * k);
-    return v;
 }
 __device__ __forceinline__ void store_u32u(uint8_t* p, uint32_t v) {
     typedef uint32_t __attribute__((aligned(1))) u32u;
@@ -234,7 +247,7 @@ __device__ __forceinline__ uint32_t load_u32u(const uint8_t* p) {
 }
 
 // ---------------------------------------------------------------------------
-// decoupled lookback (single pass, wave-parallel, ticketed tiles)
+// decoupled lookback (single pass, wave-parallel)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t ld_status(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -243,9 +256,26 @@ __device__ __forceinline__ void st_status(unsigned long long* p, uint64_t v) {
     __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Sum over the wave of per-lane values below 2^40: two 20-bit halves are
+// reduced with DPP adds (no LDS round trips) and recombined.
+__device__ __forceinline__ uint64_t wave_sum_u40(uint64_t v) {
+    const uint32_t lo = scan_wave((uint32_t)v & 0xfffffu);
+    const uint32_t hi = scan_wave((uint32_t)(v >> 20));
+    return (uint64_t)rdlane(lo, kWaveSize - 1) + ((uint64_t)rdlane(hi, kWaveSize - 1) << 20);
+}
+
+constexpr int kLookbackWords = 4;   // predecessors read per lane and round trip: 256 per wave
+
+__device__ __forceinline__ bool status_ready(uint64_t w, uint64_t tag) {
+    return ((w & ~(kFlagMask | kValueMask)) == tag) && (w & kFlagMask);
+}
+
 // Called by one whole wave. Publishes the tile aggregate, walks back over
-// predecessors 64 at a time, publishes the inclusive prefix and returns the
-// exclusive prefix (same value in every lane).
+// predecessors 256 at a time (position p = lane + 64*k is predecessor
+// tile - 1 - p), publishes the inclusive prefix and returns the exclusive
+// prefix (same value in every lane). One round trip advances the walk by up to
+// 256 tiles, so the prefix front keeps up with tiles issued 256x faster than a
+// status round trip.
 __device__ __forceinline__ uint64_t tile_lookback(const ScanArgs& sa, uint64_t tile, uint64_t agg, int lane) {
     const uint64_t tag = sa.tag;
     if (tile == 0) {
@@ -255,56 +285,101 @@ __device__ __forceinline__ uint64_t tile_lookback(const ScanArgs& sa, uint64_t t
     if (lane == 0) st_status(sa.status + tile, tag | kFlagAgg | agg);
     uint64_t excl = 0;
     int64_t hi = (int64_t)tile;
+    uint32_t spins = 0;
+    bool stalled = false;
     for (;;) {
-        const int64_t idx = hi - 1 - lane;
-        uint64_t w = tag | kFlagPrefix;   // virtual tile -1: inclusive prefix 0
-        bool ready = true;
-        if (idx >= 0) {
-            w = ld_status(sa.status + idx);
-            ready = ((w & ~(kFlagMask | kValueMask)) == tag) && (w & kFlagMask);
+        uint64_t w[kLookbackWords];
+        bool ready[kLookbackWords];
+#pragma unroll
+        for (int k = 0; k < kLookbackWords; ++k) {
+            const int64_t idx = hi - 1 - lane - 64 * k;
+            w[k] = tag | kFlagPrefix;   // before tile 0: inclusive prefix 0
+            ready[k] = true;
+            if (idx >= 0) {
+                w[k] = ld_status(sa.status + idx);
+                ready[k] = status_ready(w[k], tag);
+            }
         }
-        // wait until the lanes up to the nearest published prefix are all ready
+        // nearest published prefix P (position); wait until positions <= P are ready
+        int P;
         for (;;) {
-            const uint64_t notready = ballot(!ready);
-            const uint64_t pm = ballot(ready && (w & kFlagMask) == kFlagPrefix);
-            uint64_t need = ~0ull;
-            if (pm) {
-                const int k = __builtin_ctzll(pm);
-                need = (k == 63) ? ~0ull : ((2ull << k) - 1);
+            P = kLookbackWords * 64;
+            bool all_ready = true;
+#pragma unroll
+            for (int k = kLookbackWords - 1; k >= 0; --k) {
+                const uint64_t pm = ballot(ready[k] && (w[k] & kFlagMask) == kFlagPrefix);
+                if (pm) P = 64 * k + __builtin_ctzll(pm);
             }
-            if (!(notready & need)) break;
+#pragma unroll
+            for (int k = 0; k < kLookbackWords; ++k) {
+                const uint64_t nr = ballot(!ready[k]);
+                const int lo = 64 * k;
+                if (P >= lo + 63) {
+                    all_ready &= (nr == 0);
+                } else if (P >= lo) {
+                    all_ready &= ((nr & ((2ull << (P - lo)) - 1)) == 0);
+                }
+            }
+            if (all_ready) break;
+            if (++spins >= kSpinLimit ||
+                ((spins & 1023) == 0 && (__hip_atomic_load(sa.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kErrStall))) {
+                // watchdog: record the first stuck predecessor and give up (the
+                // stream is reported invalid; publishing below unblocks successors)
+                stalled = true;
+                uint64_t first = ~0ull, seen = 0;
+#pragma unroll
+                for (int k = 0; k < kLookbackWords; ++k) {
+                    const uint64_t nr = ballot(!ready[k]);
+                    if (nr && first == ~0ull) {
+                        const int ln = __builtin_ctzll(nr);
+                        first = (uint64_t)(hi - 1 - ln - 64 * k);
+                        seen = __builtin_amdgcn_readlane((unsigned)w[k], ln) | ((uint64_t)__builtin_amdgcn_readlane((unsigned)(w[k] >> 32), ln) << 32);
+                    }
+                }
+                if (lane == 0) {
+                    atomicOr(sa.err, kErrStall);
+                    if (atomicCAS(sa.diag, 0ull, tile + 1) == 0ull) {
+                        sa.diag[1] = first;
+                        sa.diag[2] = seen;
+                    }
+                }
+                break;
+            }
             __builtin_amdgcn_s_sleep(1);
-            if (!ready) {
-                w = ld_status(sa.status + idx);
-                ready = ((w & ~(kFlagMask | kValueMask)) == tag) && (w & kFlagMask);
+#pragma unroll
+            for (int k = 0; k < kLookbackWords; ++k) {
+                const int64_t idx = hi - 1 - lane - 64 * k;
+                if (!ready[k]) {
+                    w[k] = ld_status(sa.status + idx);
+                    ready[k] = status_ready(w[k], tag);
+                }
             }
         }
-        const uint64_t pm = ballot((w & kFlagMask) == kFlagPrefix && ready);
-        if (pm) {
-            const int k = __builtin_ctzll(pm);
-            excl += wave_sum_u64(lane <= k ? (w & kValueMask) : 0);
-            break;
-        }
-        excl += wave_sum_u64(w & kValueMask);
-        hi -= kWaveSize;
+        uint64_t part = 0;
+#pragma unroll
+        for (int k = 0; k < kLookbackWords; ++k) part += (lane + 64 * k <= P) ? (w[k] & kValueMask) : 0;
+        excl += wave_sum_u40(part);
+        if (P < kLookbackWords * 64 || stalled) break;
+        hi -= kLookbackWords * 64;
     }
     if (lane == 0) st_status(sa.status + tile, tag | kFlagPrefix | (excl + agg));
     return excl;
 }
 
-// Workgroup-level: combine the 4 wave totals, run the lookback on wave 0 and
+// Workgroup-level: combine the wave totals, run the lookback on wave 0 and
 // return {wave's exclusive offset within the stream payload, tile inclusive}.
 struct TileOffsets {
     uint64_t wave_base;   // payload offset (excluding H) of this wave's first block
     uint64_t tile_incl;   // payload bytes up to and including this tile
 };
+template <int WAVES>
 __device__ __forceinline__ TileOffsets tile_scan(const ScanArgs& sa, uint64_t tile, uint64_t wtot,
                                                  int wave, int lane, uint64_t* s_wtot, uint64_t* s_excl) {
     if (lane == 0) s_wtot[wave] = wtot;
     __syncthreads();
     uint64_t wexcl = 0, ttot = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
+    for (int w = 0; w < WAVES; ++w) {
         const uint64_t v = s_wtot[w];
         wexcl += (w < wave) ? v : 0;
         ttot += v;
@@ -316,20 +391,6 @@ __device__ __forceinline__ TileOffsets tile_scan(const ScanArgs& sa, uint64_t ti
     __syncthreads();
     const uint64_t excl = *s_excl;
     return TileOffsets{excl + wexcl, excl + ttot};
-}
-
-// Tiles are numbered in the order workgroups start, so a tile only ever waits
-// on tiles whose workgroups are already resident. The workgroup that draws the
-// last ticket returns the counter to 0 for the next launch on this scan state:
-// every other draw of this launch precedes its own in the counter's order.
-__device__ __forceinline__ uint64_t claim_tile(const ScanArgs& sa, uint64_t ntiles, uint64_t* s_tile) {
-    if (threadIdx.x == 0) {
-        const uint64_t t = __hip_atomic_fetch_add(sa.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (t == ntiles - 1) __hip_atomic_store(sa.ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *s_tile = t;
-    }
-    __syncthreads();
-    return *s_tile;
 }
 
 // ---------------------------------------------------------------------------
@@ -444,42 +505,100 @@ __device__ __forceinline__ uint32_t inverse_lorenzo(uint32_t d, int e) {
 }
 
 // ---------------------------------------------------------------------------
+// per-block metadata handed from the bookkeeping lanes to pass 2 through LDS:
+//   [19:0] payload offset from the wave's first block, [25:20] planes (<= 32),
+//   [26] block has a payload, [27] outlier-selected, [30:28] prefix bytes
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pack_meta(uint32_t off, uint32_t R, bool nz, uint32_t sel, uint32_t head) {
+    return off | ((R < 32 ? R : 32u) << 20) | ((uint32_t)nz << 26) | (sel << 27) | (head << 28);
+}
+struct Meta {
+    uint32_t off, R, nz, sel, head;
+    __device__ __forceinline__ explicit Meta(uint32_t m)
+        : off(m & 0xfffff), R((m >> 20) & 63), nz((m >> 26) & 1), sel((m >> 27) & 1), head((m >> 28) & 7) {}
+};
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// Byte offset of lane (l, h)'s word inside a block payload (after the outlier
+// prefix): lane 31 carries the sign word, lane l < 31 plane l; 2D/3D blocks
+// interleave the two 32-element halves h word by word (src/cuSZp_kernels_2D_f32.cu:896-903).
+template <int DIM>
+__device__ __forceinline__ uint32_t word_offset(uint32_t l, uint32_t h, bool sign_in_lane31) {
+    if constexpr (DIM == 1) {
+        return (l == 31) ? 0u : 4u + 4u * l;
+    } else {
+        return (sign_in_lane31 && l == 31) ? 4u * h : 8u + 8u * l + 4u * h;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // compress
 // ---------------------------------------------------------------------------
-template <int DIM, int MODE, class T, int G>
-__global__ void __launch_bounds__(kThreads)
+template <int DIM, int MODE, class T, int G, int WAVES>
+__global__ void __launch_bounds__(WAVES * kWaveSize)
 compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom geo, const T r,
                 const ScanArgs sa, const uint64_t cap) {
-    constexpr int S = (DIM == 1) ? 4 : 8;            // bytes per plane
-    __shared__ uint64_t s_tile, s_excl;
-    __shared__ uint64_t s_wtot[kWaves];
-    __shared__ uint32_t s_aux[kWaves][G][2];         // outlier prefix values / top planes
+    constexpr int S = (DIM == 1) ? 4 : 8;        // bytes per plane
+    constexpr int BPG = (DIM == 1) ? 2 : 1;      // blocks per group
+    constexpr int NB = BPG * G;                  // blocks per wave
+    static_assert(NB <= kWaveSize, "one bookkeeping lane per block");
+    __shared__ uint64_t s_excl;
+    __shared__ uint64_t s_wtot[WAVES];
+    __shared__ uint32_t s_meta[WAVES][NB];
+    __shared__ uint32_t s_sgn[WAVES][G][2];     // sign words of 2D/3D blocks holding |d| = 2^31
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint64_t tile = claim_tile(sa, geo.ntiles, &s_tile);
-    if (tile >= geo.ntiles) return;   // unreachable: exactly ntiles draws per launch
-    const uint64_t grp0 = (tile * kWaves + (uint64_t)wave) * G;
+    const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
 
     LaneGeo<DIM> lg;
     lg.init(lane, geo);
-    Walker wk;
-    if constexpr (DIM != 1) wk.init(grp0, geo);
-    const int plane = pred_lane<DIM>(lane);
-    const int plane_src = plane < 0 ? lane : plane;
-    const bool is_e0 = (DIM == 1) ? ((lane & 31) == 7) : (lane == 7);   // element 0, mirror order
+    const int pl = pred_lane<DIM>(lane);
+    const int pl_src = pl < 0 ? lane : pl;
+    XposeConsts xc;
+    xc.init(lane);
+    // after the transpose, element 0 of a block is bit 7 of its plane words: in
+    // both halves for 1D (two blocks), only in the low half for 2D/3D
+    const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
+
+    // All loads of a tile's G groups per wave are issued back to back; the next
+    // tile's loads are issued before the current tile's scan and stores.
+    T xv[G];
+    uint64_t vmask = 0;   // bit g: this lane's element of group g lies inside the array
+    auto load_tile = [&](uint64_t tl) {
+        const uint64_t g0 = (tl * WAVES + (uint64_t)wave) * G;
+        Walker wk;
+        if constexpr (DIM != 1) wk.init(g0, geo);
+        vmask = 0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            uint64_t idx;
+            const bool valid = elem_of<DIM>(geo, lg, wk, g0 + g, lane, idx);
+            xv[g] = valid ? __builtin_nontemporal_load(in + idx) : T(0);
+            vmask |= (uint64_t)valid << g;
+            if constexpr (DIM != 1) wk.next(geo);
+        }
+    };
+    const uint64_t stride = gridDim.x;
+    uint64_t tile = blockIdx.x;
+    if (tile < geo.ntiles) load_tile(tile);
+    for (; tile < geo.ntiles; tile += stride) {
+    const uint64_t grp0 = (tile * WAVES + (uint64_t)wave) * G;
 
     uint32_t W[G];
-    uint32_t hv = 0;  // lane g: header byte(s) of group g
-    uint64_t wtot = 0;
-
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const uint64_t grp = grp0 + g;
-        uint64_t idx;
-        const bool valid = elem_of<DIM>(geo, lg, wk, grp, lane, idx);
-        uint32_t q = 0;
-        if (valid) q = quantize<T>(in[idx], r);
+    uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
+    uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
+    static_for<0, G>([&](auto gc) {
+        constexpr int g = decltype(gc)::value;
+        const bool valid = (vmask >> g) & 1;
+        const uint32_t q = quantize<T>(xv[g], r);   // x = 0 outside the array gives q = 0
         uint32_t d;
         if constexpr (MODE == kFixed) {
             d = q;
@@ -488,233 +607,198 @@ compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom 
             qp = (lane & 31) ? qp : 0u;
             d = q - qp;
         } else {
-            uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(plane_src << 2, (int)q);
-            qp = (plane >= 0) ? qp : 0u;
+            uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
+            qp = (pl >= 0) ? qp : 0u;
             d = q - qp;
         }
         d = valid ? d : 0u;
-        // mirror order: lane L now holds element L^7
-        const uint32_t dm = dpp<kDppHalfMirror>(d);
-        const uint64_t sgn = ballot((int32_t)dm < 0);
+        const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
         const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
-        // 1D ranks |INT_MIN| as 0 (signed running max, src/cuSZp_kernels_1D_f32.cu:552)
-        const uint32_t ra = (DIM == 1) ? (a & 0x7fffffffu) : a;
-        const uint32_t orow = row_or(is_e0 ? 0u : ra);
-        const uint32_t o0 = rdlane(orow, 0), o1 = rdlane(orow, 16), o2 = rdlane(orow, 32), o3 = rdlane(orow, 48);
-
-        uint32_t bsrc;
-        uint32_t w = 0;
-        if constexpr (DIM == 1) {
-            const uint32_t oA = rdlane(a, 7), oB = rdlane(a, 39);
-            const uint32_t byteA = rate_byte<MODE, 4>(o0 | o1, oA & 0x7fffffffu, oA);
-            const uint32_t byteB = rate_byte<MODE, 4>(o2 | o3, oB & 0x7fffffffu, oB);
-            const BlockView vA = view_of<MODE, 4>(byteA);
-            const BlockView vB = view_of<MODE, 4>(byteB);
-            hv = wrlane(byteA | (byteB << 8), g, hv);
-            wtot += vA.size + vB.size;
-            if constexpr (MODE == kOutlier) {
-                if (lane == 0) { s_aux[wave][g][0] = oA; s_aux[wave][g][1] = oB; }
+        uint64_t m1, m2 = 0;
+        bool wide = false;
+        if constexpr (DIM != 1) wide = ballot((int32_t)a < 0) != 0;
+        if (!wide) {
+            // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
+            // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
+            const uint32_t w = xpose32(bfi(0x80000000u, dm, a), xc);
+            W[g] = w;
+            m1 = ballot(w != 0) & 0x7fffffff7fffffffull;
+            if constexpr (MODE == kOutlier) m2 = ballot((w & not_e0) != 0) & 0x7fffffff7fffffffull;
+        } else {
+            const uint32_t w = xpose32(a, xc);
+            W[g] = w;
+            m1 = ballot(w != 0);
+            if constexpr (MODE == kOutlier) m2 = ballot((w & not_e0) != 0);
+            const uint64_t sg = ballot((int32_t)dm < 0);
+            if (lane == 0) {
+                s_sgn[wave][g][0] = (uint32_t)sg;
+                s_sgn[wave][g][1] = (uint32_t)(sg >> 32);
             }
-            const bool sel = (lane < 32) ? vA.sel : vB.sel;
-            bsrc = __builtin_bitreverse32((sel && is_e0) ? 0u : ra);
-            w = wrlane((uint32_t)sgn, 0, w);
-            w = wrlane((uint32_t)(sgn >> 32), 32, w);
-            const uint32_t Rmax = vA.R > vB.R ? vA.R : vB.R;
-            for (uint32_t t = 0; t < Rmax; ++t) {
-                const uint64_t m = take_top_bit(bsrc);
-                w = wrlane((uint32_t)m, 1 + t, w);
-                w = wrlane((uint32_t)(m >> 32), 33 + t, w);
+            big |= 1u << g;
+        }
+        if constexpr (DIM == 1) {
+            bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
+            bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
+            if constexpr (MODE == kOutlier) {
+                bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
+                bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
+                bko = wrlane<2 * g>(rdlane(a, 7), bko);
+                bko = wrlane<2 * g + 1>(rdlane(a, 39), bko);
             }
         } else {
-            const uint32_t o = rdlane(a, 7);
-            const uint32_t byte = rate_byte<MODE, 8>(o0 | o1 | o2 | o3, o, o);
-            const BlockView v = view_of<MODE, 8>(byte);
-            hv = wrlane(byte, g, hv);
-            wtot += v.size;
-            bsrc = __builtin_bitreverse32((v.sel && is_e0) ? 0u : a);
-            w = wrlane((uint32_t)sgn, 0, w);
-            w = wrlane((uint32_t)(sgn >> 32), 1, w);
-            const uint32_t R31 = v.R < 31 ? v.R : 31;
-            for (uint32_t t = 0; t < R31; ++t) {
-                const uint64_t m = take_top_bit(bsrc);
-                w = wrlane((uint32_t)m, 2 + 2 * t, w);
-                w = wrlane((uint32_t)(m >> 32), 3 + 2 * t, w);
-            }
+            bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
             if constexpr (MODE == kOutlier) {
-                if (lane == 0) s_aux[wave][g][0] = o;
-            } else {
-                if (v.R == 32) {   // only when a residual is -2^31: plane 31 lives in LDS
-                    const uint64_t m = take_top_bit(bsrc);
-                    if (lane == 0) { s_aux[wave][g][0] = (uint32_t)m; s_aux[wave][g][1] = (uint32_t)(m >> 32); }
-                }
-            }
-            wk.next(geo);
-        }
-        W[g] = w;
-    }
-
-    // header bytes of this wave's groups
-    {
-        if (lane < G) {
-            const uint64_t grp = grp0 + lane;
-            if constexpr (DIM == 1) {
-                typedef uint16_t __attribute__((aligned(1))) u16u;
-                *(u16u*)(out + 2 * grp) = (uint16_t)hv;
-            } else {
-                if (grp < geo.H) out[grp] = (uint8_t)hv;
+                bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
+                bko = wrlane<g>(rdlane(a, 7), bko);
             }
         }
+    });
+
+    if (tile + stride < geo.ntiles) load_tile(tile + stride);   // in flight during scan + stores
+
+    // ---- bookkeeping: lane j <-> block j of this wave -----------------------
+    const uint64_t blk_idx = (DIM == 1) ? 2 * grp0 + (uint64_t)lane : grp0 + (uint64_t)lane;
+    const bool blk = lane < NB;
+    uint32_t byte;
+    if constexpr (MODE == kOutlier) byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
+    else byte = bitlen(bk1);
+    const BlockView v = view_of<MODE, S>(byte);
+    const uint32_t sz = blk ? v.size : 0u;
+    const uint32_t incl = scan_wave(sz);
+    const uint32_t off = incl - sz;
+    const uint64_t wtot = rdlane(incl, kWaveSize - 1);
+    if (blk) {
+        s_meta[wave][lane] = pack_meta(off, v.R, sz != 0, v.sel, v.head);
+        // header byte; 2D/3D pad bytes [nblocks, H) are written as 0
+        if (DIM == 1 || blk_idx < geo.H) out[blk_idx] = (uint8_t)byte;
     }
 
-    const TileOffsets to = tile_scan(sa, tile, wtot, wave, lane, s_wtot, &s_excl);
+    const TileOffsets to = tile_scan<WAVES>(sa, tile, wtot, wave, lane, s_wtot, &s_excl);
     if (tile == geo.ntiles - 1 && threadIdx.x == 0 && sa.total) *sa.total = geo.H + to.tile_incl;
-
-    uint64_t base = geo.H + to.wave_base;
-    if (base + wtot > cap) {   // never write past the caller's buffer
+    uint8_t* const wbase = out + geo.H + to.wave_base;
+    if (geo.H + to.wave_base + wtot > cap) {   // never write past the caller's buffer
         if (lane == 0) atomicOr(sa.err, kErrOverflow);
-        return;
+        continue;
     }
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        if constexpr (DIM == 1) {
-            const uint32_t hg = rdlane(hv, g);
-            const BlockView vA = view_of<MODE, 4>(hg & 0xff);
-            const BlockView vB = view_of<MODE, 4>(hg >> 8);
-            const uint64_t offA = base, offB = base + vA.size;
-            base += vA.size + vB.size;
-            const uint32_t l = (uint32_t)lane & 31;
-            const bool hiH = lane >= 32;
-            const uint32_t nw = hiH ? vB.nwords : vA.nwords;
-            const uint64_t wo = hiH ? offB + vB.head : offA + vA.head;
-            if (l < nw) store_u32u(out + wo + 4 * l, W[g]);
-            if constexpr (MODE == kOutlier) {
-                if (lane == 0 && vA.sel) store_le(out + offA, s_aux[wave][g][0], vA.nb);
-                if (lane == 32 && vB.sel) store_le(out + offB, s_aux[wave][g][1], vB.nb);
-            }
-        } else {
-            const BlockView v = view_of<MODE, 8>(rdlane(hv, g));
-            const uint64_t off = base;
-            base += v.size;
-            const uint32_t nw = v.nwords < 64 ? v.nwords : 64;
-            if ((uint32_t)lane < nw) store_u32u(out + off + v.head + 4 * lane, W[g]);
-            if constexpr (MODE == kOutlier) {
-                if (lane == 0 && v.sel) store_le(out + off, s_aux[wave][g][0], v.nb);
-            } else {
-                if (v.R == 32 && lane < 2) store_u32u(out + off + 256 + 4 * lane, s_aux[wave][g][lane]);
-            }
+    if constexpr (MODE == kOutlier) {
+        if (blk && v.sel) store_le(wbase + off, bko, v.nb);   // little-endian |d| of element 0
+    }
+
+    // ---- pass 2: stream words ----------------------------------------------
+    static_for<0, G>([&](auto gc) {
+        constexpr int g = decltype(gc)::value;
+        const Meta m(s_meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
+        const bool wideg = (DIM != 1) && ((big >> g) & 1);
+        uint32_t w = W[g];
+        if constexpr (MODE == kOutlier) {
+            if (m.sel && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
         }
-    }
+        const bool st = m.nz && (l < m.R || (l == 31 && !wideg));
+        if (st) store_u32u(wbase + m.off + m.head + word_offset<DIM>(l, h, !wideg), w);
+        if constexpr (DIM != 1) {
+            if (wideg && m.nz && lane < 2) store_u32u(wbase + m.off + m.head + 4 * lane, s_sgn[wave][g][lane]);
+        }
+    });
+    }   // tiles
 }
 
 // ---------------------------------------------------------------------------
 // decompress
 // ---------------------------------------------------------------------------
-template <int DIM, int MODE, class T, int G>
-__global__ void __launch_bounds__(kThreads)
+template <int DIM, int MODE, class T, int G, int WAVES>
+__global__ void __launch_bounds__(WAVES * kWaveSize)
 decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Geom geo, const T eb,
                   const ScanArgs sa, const uint64_t cmp_size) {
     constexpr int S = (DIM == 1) ? 4 : 8;
-    __shared__ uint64_t s_tile, s_excl;
-    __shared__ uint64_t s_wtot[kWaves];
+    constexpr int BPG = (DIM == 1) ? 2 : 1;
+    constexpr int NB = BPG * G;
+    __shared__ uint64_t s_excl;
+    __shared__ uint64_t s_wtot[WAVES];
+    __shared__ uint32_t s_meta[WAVES][NB];
+    __shared__ uint32_t s_o[WAVES][NB];
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint64_t tile = claim_tile(sa, geo.ntiles, &s_tile);
-    if (tile >= geo.ntiles) return;   // unreachable: exactly ntiles draws per launch
-    const uint64_t grp0 = (tile * kWaves + (uint64_t)wave) * G;
-
-    // header bytes and block sizes of this wave's groups: lane g <-> group g
-    uint32_t hv = 0, sz = 0;
-    if (lane < G) {
-        const uint64_t grp = grp0 + lane;
-        if constexpr (DIM == 1) {
-            typedef uint16_t __attribute__((aligned(1))) u16u;
-            hv = *(const u16u*)(cmp + 2 * grp);
-            sz = view_of<MODE, 4>(hv & 0xff).size + view_of<MODE, 4>(hv >> 8).size;
-        } else {
-            if (grp < geo.nblocks) {
-                hv = cmp[grp];
-                sz = view_of<MODE, 8>(hv).size;
-            }
-        }
-    }
-    const uint32_t incl = scan_wave(sz);
-    const uint32_t pre = incl - sz;
-    const uint64_t wtot = rdlane(incl, 63);
-    const TileOffsets to = tile_scan(sa, tile, wtot, wave, lane, s_wtot, &s_excl);
-    if (tile == geo.ntiles - 1 && threadIdx.x == 0 && sa.total) *sa.total = geo.H + to.tile_incl;
-    const uint64_t base = geo.H + to.wave_base;
-    const bool intact = base + wtot <= cmp_size;
-    if (!intact && lane == 0) atomicOr(sa.err, kErrTruncated);
-
+    const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
     LaneGeo<DIM> lg;
     lg.init(lane, geo);
+    XposeConsts xc;
+    xc.init(lane);
+
+    for (uint64_t tile = blockIdx.x; tile < geo.ntiles; tile += gridDim.x) {
+    const uint64_t grp0 = (tile * WAVES + (uint64_t)wave) * G;
+
+    // ---- bookkeeping: lane j <-> block j of this wave -----------------------
+    const uint64_t blk_idx = (DIM == 1) ? 2 * grp0 + (uint64_t)lane : grp0 + (uint64_t)lane;
+    const bool blk = (lane < NB) && (DIM == 1 || blk_idx < geo.nblocks);
+    const uint32_t byte = blk ? (uint32_t)cmp[blk_idx] : 0u;
+    const BlockView v = view_of<MODE, S>(byte);
+    const uint32_t sz = blk ? v.size : 0u;
+    const uint32_t incl = scan_wave(sz);
+    const uint32_t off = incl - sz;
+    const uint64_t wtot = rdlane(incl, kWaveSize - 1);
+    if (blk) s_meta[wave][lane] = pack_meta(off, v.R, sz != 0, v.sel, v.head);
+
+    const TileOffsets to = tile_scan<WAVES>(sa, tile, wtot, wave, lane, s_wtot, &s_excl);
+    if (tile == geo.ntiles - 1 && threadIdx.x == 0 && sa.total) *sa.total = geo.H + to.tile_incl;
+    const uint8_t* const wbase = cmp + geo.H + to.wave_base;
+    const bool intact = geo.H + to.wave_base + wtot <= cmp_size;
+    if (!intact && lane == 0) atomicOr(sa.err, kErrTruncated);
+    if constexpr (MODE == kOutlier) {
+        // a selected block starts with nb <= 4 prefix bytes and then a 4-byte sign
+        // word, so one u32 read at the block start stays inside the block
+        uint32_t o = 0;
+        if (blk && intact && v.sel) {
+            o = load_u32u(wbase + off);
+            if (v.nb < 4) o &= (1u << (8 * v.nb)) - 1;
+        }
+        if (lane < NB) s_o[wave][lane] = o;
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // ---- prefetch the stream words of all G groups ---------------------------
+    uint32_t W[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const Meta m(s_meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
+        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.R >= 32;
+        const bool ld = intact && m.nz && (l < m.R || (l == 31 && !wideg));
+        W[g] = ld ? load_u32u(wbase + m.off + m.head + word_offset<DIM>(l, h, !wideg)) : 0u;
+    }
+
+    // ---- decode ---------------------------------------------------------------
     Walker wk;
     if constexpr (DIM != 1) wk.init(grp0, geo);
-
-#pragma unroll 2
+#pragma unroll
     for (int g = 0; g < G; ++g) {
-        const uint64_t grp = grp0 + g;
-        const uint32_t hg = rdlane(hv, g);
-        const uint64_t goff = base + rdlane(pre, g);
-        uint32_t a = 0, sgnw0, sgnw1;
-        bool zero_blk;
-        if constexpr (DIM == 1) {
-            const BlockView vA = view_of<MODE, 4>(hg & 0xff);
-            const BlockView vB = view_of<MODE, 4>(hg >> 8);
-            const uint64_t offA = goff, offB = goff + vA.size;
-            const uint32_t l = (uint32_t)lane & 31;
-            const bool hiH = lane >= 32;
-            const uint32_t nw = intact ? (hiH ? vB.nwords : vA.nwords) : 0;
-            const uint64_t wo = hiH ? offB + vB.head : offA + vA.head;
-            const uint32_t w = (l < nw) ? load_u32u(cmp + wo + 4 * l) : 0u;
-            sgnw0 = rdlane(w, 0);
-            sgnw1 = rdlane(w, 32);
-            const uint32_t Rmax = vA.R > vB.R ? vA.R : vB.R;
-            for (int32_t t = (int32_t)Rmax - 1; t >= 0; --t) {
-                const uint32_t lo = ((uint32_t)t < vA.R) ? rdlane(w, 1 + t) : 0u;
-                const uint32_t hi = ((uint32_t)t < vB.R) ? rdlane(w, 33 + t) : 0u;
-                a = push_bit(a, (uint64_t)lo | ((uint64_t)hi << 32));
-            }
-            if constexpr (MODE == kOutlier) {
-                if (intact && vA.sel && lane == 7) a = load_le(cmp + offA, vA.nb);
-                if (intact && vB.sel && lane == 39) a = load_le(cmp + offB, vB.nb);
-            }
-            zero_blk = !intact || (hiH ? vB.size == 0 : vA.size == 0);
-        } else {
-            const BlockView v = view_of<MODE, 8>(hg);
-            const uint32_t nw = intact ? v.nwords : 0;
-            const uint64_t wo = goff + v.head;
-            const uint32_t w = ((uint32_t)lane < nw) ? load_u32u(cmp + wo + 4 * lane) : 0u;
-            sgnw0 = rdlane(w, 0);
-            sgnw1 = rdlane(w, 1);
-            if constexpr (MODE != kOutlier) {
-                if (v.R == 32 && nw) {
-                    const uint32_t w2 = (lane < 2) ? load_u32u(cmp + wo + 256 + 4 * lane) : 0u;
-                    a = push_bit(a, (uint64_t)rdlane(w2, 0) | ((uint64_t)rdlane(w2, 1) << 32));
-                }
-            }
-            const uint32_t R31 = v.R < 31 ? v.R : 31;
-            for (int32_t t = (int32_t)R31 - 1; t >= 0; --t) {
-                a = push_bit(a, (uint64_t)rdlane(w, 2 + 2 * t) | ((uint64_t)rdlane(w, 3 + 2 * t) << 32));
-            }
-            if constexpr (MODE == kOutlier) {
-                if (intact && v.sel && lane == 7) a = load_le(cmp + goff, v.nb);
-            }
-            zero_blk = !intact || v.size == 0;
+        const Meta m(s_meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
+        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.R >= 32;
+        const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
+        uint32_t a, neg;
+        if (!wideg) {
+            neg = x >> 31;
+            a = x & 0x7fffffffu;
+        } else {   // 32 planes: the sign words sit in their own two words
+            const uint32_t sw = (intact && m.nz) ? load_u32u(wbase + m.off + 4 * h) : 0u;
+            neg = (sw >> l) & 1;
+            a = x;
         }
-        const uint64_t sgn = (uint64_t)sgnw0 | ((uint64_t)sgnw1 << 32);
-        const uint32_t dm = ((sgn >> lane) & 1) ? 0u - a : a;
-        const uint32_t d = dpp<kDppHalfMirror>(dm);          // back to natural order
+        if constexpr (MODE == kOutlier) {
+            if (m.sel && l == 7 && (DIM == 1 || h == 0)) a = s_o[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
+        }
+        const uint32_t dm = neg ? 0u - a : a;
+        const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
         uint32_t q;
         if constexpr (MODE == kFixed) q = d;
         else q = inverse_lorenzo<DIM>(d, lane);
+        const bool zero_blk = !intact || !m.nz;
         const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
         uint64_t idx;
-        const bool valid = elem_of<DIM>(geo, lg, wk, grp, lane, idx);
+        const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
         if (valid) out[idx] = val;
         if constexpr (DIM != 1) wk.next(geo);
     }
+    }   // tiles
 }
 
 }  // namespace cuszp

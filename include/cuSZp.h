@@ -64,7 +64,8 @@ typedef enum {
     CUSZP_ERROR_TOO_LARGE = 3,          /* stream or block count beyond the format's limits */
     CUSZP_ERROR_OUTPUT_TOO_SMALL = 4,   /* compressed stream exceeds cmpCapacity */
     CUSZP_ERROR_CORRUPT_STREAM = 5,     /* cmpSize shorter than the header describes */
-    CUSZP_ERROR_HIP = 6                 /* a HIP runtime call failed */
+    CUSZP_ERROR_HIP = 6,                /* a HIP runtime call failed */
+    CUSZP_ERROR_INTERNAL = 7            /* device-side scan watchdog fired; output invalid */
 } cuszp_status_t;
 
 /* ---- generic entries: include/cuSZp.h:38-45 (reference) ---------------- */

@@ -99,6 +99,6 @@ def test_zero_elements_is_a_noop(L):
 
 
 def test_status_strings(L):
-    for s in range(7):
+    for s in range(8):
         assert L.cuSZp_status_string(s)
     assert b"gfx950" in L.cuSZp_version()
