@@ -37,8 +37,34 @@ EXPORTED = (
         "cuSZp_last_async_status",
         "cuSZp_status_string",
         "cuSZp_version",
+        "cuSZp_batch_plan_create",
+        "cuSZp_batch_plan_execute",
+        "cuSZp_batch_plan_results",
+        "cuSZp_batch_plan_device_sizes",
+        "cuSZp_batch_plan_tune",
+        "cuSZp_batch_plan_destroy",
+        "cuSZp_compress_batch",
+        "cuSZp_decompress_batch",
     ]
 )
+
+# cuszp_batch_item_t (include/cuSZp.h): 64 bytes
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("d_in", ctypes.c_void_p),
+        ("d_out", ctypes.c_void_p),
+        ("nbEle", ctypes.c_size_t),
+        ("capacity", ctypes.c_size_t),
+        ("errorBound", ctypes.c_double),
+        ("dim", ctypes.c_int),
+        ("dims", Uint3),
+        ("type", ctypes.c_int),
+        ("mode", ctypes.c_int),
+    ]
+
+
+BATCH_COMPRESS, BATCH_DECOMPRESS = 0, 1
+BATCH_GRAPH, BATCH_TUNE = 1, 2
 
 
 def load() -> ctypes.CDLL:
@@ -85,5 +111,22 @@ def load() -> ctypes.CDLL:
     L.cuSZp_status_string.restype = ctypes.c_char_p
     L.cuSZp_version.argtypes = []
     L.cuSZp_version.restype = ctypes.c_char_p
+    pitems = ctypes.POINTER(BatchItem)
+    L.cuSZp_batch_plan_create.argtypes = [ctypes.POINTER(vp), pitems, sz, i32, i32, vp, psz]
+    L.cuSZp_batch_plan_create.restype = i32
+    L.cuSZp_batch_plan_execute.argtypes = [vp, vp]
+    L.cuSZp_batch_plan_execute.restype = i32
+    L.cuSZp_batch_plan_results.argtypes = [vp, psz, ctypes.POINTER(i32), vp]
+    L.cuSZp_batch_plan_results.restype = i32
+    L.cuSZp_batch_plan_device_sizes.argtypes = [vp]
+    L.cuSZp_batch_plan_device_sizes.restype = vp
+    L.cuSZp_batch_plan_tune.argtypes = [vp, vp, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_uint)]
+    L.cuSZp_batch_plan_tune.restype = i32
+    L.cuSZp_batch_plan_destroy.argtypes = [vp]
+    L.cuSZp_batch_plan_destroy.restype = None
+    L.cuSZp_compress_batch.argtypes = [pitems, sz, psz, ctypes.POINTER(i32), vp]
+    L.cuSZp_compress_batch.restype = i32
+    L.cuSZp_decompress_batch.argtypes = [pitems, sz, ctypes.POINTER(i32), vp]
+    L.cuSZp_decompress_batch.restype = i32
     _lib = L
     return L

@@ -1,0 +1,105 @@
+// Argument validation, geometry and item construction shared by the
+// single-array C ABI (cuszp_api.cpp) and the batched one (cuszp_batch.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "../../include/cuSZp.h"
+#include "launch.hpp"
+
+namespace cuszp {
+
+constexpr uint64_t kMaxStream = (1ull << 38) - 1;   // status-word value field
+
+inline int status_from_bits(unsigned long long bits) {
+    bits &= kErrMask;
+    if (bits & kErrStall) return CUSZP_ERROR_INTERNAL;
+    if (bits & kErrOverflow) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
+    if (bits & kErrTruncated) return CUSZP_ERROR_CORRUPT_STREAM;
+    return CUSZP_SUCCESS;
+}
+
+// A launch in which some workgroup had to compute a predecessor's byte count
+// ran a larger persistent grid than the device kept resident: use a smaller
+// one from now on (the launch itself completed correctly).
+inline void note_steal(unsigned long long bits) {
+    if (!(bits & kInfoStolen)) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    int& q = grid_sixteenths(dev);
+    if (q > 4) --q;
+}
+
+// Geometry of a (dim, nbEle, dims) triple; mirrors the reference's block counts.
+inline int make_geom(int dim, size_t n, cuszp_dims_t d, Geom& g) {
+    std::memset(&g, 0, sizeof(g));
+    if (dim == CUSZP_DIM_1D) {
+        g.n = n;
+        g.nblocks = (n + 31) / 32;
+        g.H = (n + 32767) / 32768 * 1024;          // src/cuSZp_entry_1D_f32.cu:43
+        return CUSZP_SUCCESS;
+    }
+    if (dim != CUSZP_DIM_2D && dim != CUSZP_DIM_3D) return CUSZP_ERROR_INVALID_ARGUMENT;
+    if (!d.x || !d.y || !d.z || (d.x % 4)) return CUSZP_ERROR_INVALID_DIMS;   // float4 loads in the reference
+    const uint64_t xyz = (uint64_t)d.x * d.y * d.z;
+    if ((uint64_t)n != xyz) return CUSZP_ERROR_INVALID_DIMS;
+    g.n = n;
+    g.X = d.x; g.Y = d.y; g.Z = d.z;
+    g.XY = (uint64_t)d.x * d.y;
+    uint64_t bz;
+    if (dim == CUSZP_DIM_2D) {                      // src/cuSZp_entry_2D_f32.cu:21-23
+        g.Bx = (d.x + 7) / 8; g.By = (d.y + 7) / 8; bz = d.z;
+    } else {                                        // src/cuSZp_entry_3D_f32.cu:21-24
+        g.Bx = (d.x + 3) / 4; g.By = (d.y + 3) / 4; bz = (d.z + 3) / 4;
+    }
+    g.nblocks = (uint64_t)g.Bx * g.By * bz;
+    if (g.nblocks >= (1ull << 40)) return CUSZP_ERROR_TOO_LARGE;
+    g.H = (g.nblocks + 3) / 4 * 4;                  // src/cuSZp_kernels_2D_f32.cu:46
+    return CUSZP_SUCCESS;
+}
+
+inline uint64_t max_stream_bytes(int dim, const Geom& g) {
+    const uint64_t S = (dim == CUSZP_DIM_1D) ? 4 : 8;
+    return g.H + g.nblocks * (S + 32 * S + 4);
+}
+
+inline bool bad_enums(int dim, int type, int mode) {
+    return (dim < 1 || dim > 3) || (type != CUSZP_TYPE_FLOAT && type != CUSZP_TYPE_DOUBLE) ||
+           (mode < CUSZP_MODE_FIXED || mode > CUSZP_MODE_OUTLIER);
+}
+
+inline bool bad_eb(double eb, int type) {
+    if (type == CUSZP_TYPE_FLOAT) {
+        const float f = (float)eb;
+        return !(f > 0.0f) || !std::isfinite(f);
+    }
+    return !(eb > 0.0) || !std::isfinite(eb);
+}
+
+inline double decompress_param(double eb, int type) { return type == CUSZP_TYPE_FLOAT ? (double)(float)eb : eb; }
+inline double compress_param(double eb, int type) {
+    // 0.5f/eb rounded in the array's type (src/cuSZp_kernels_1D_f32.cu:48, _1D_f64.cu:33)
+    if (type == CUSZP_TYPE_FLOAT) return (double)(0.5f / (float)eb);
+    return 0.5 / eb;
+}
+
+inline Item make_item(const void* src, void* dst, uint64_t limit, unsigned long long* total, unsigned int* err,
+               double param, int dim, int type, int mode, const Geom& g) {
+    Item it;
+    std::memset(&it, 0, sizeof(it));
+    it.src = src;
+    it.dst = dst;
+    it.limit = limit;
+    it.total = total;
+    it.err = err;
+    it.param = param;
+    it.tile0 = 0;
+    it.variant = (uint32_t)(type * 9 + (dim - 1) * 3 + mode);
+    it.geo = g;
+    return it;
+}
+
+}  // namespace cuszp

@@ -23,12 +23,11 @@
 #include <utility>
 
 #include "../../include/cuSZp.h"
-#include "launch.hpp"
+#include "common.hpp"
 
 namespace cuszp {
 namespace {
 
-constexpr uint64_t kMaxStream = (1ull << 38) - 1;   // status-word value field
 
 struct Slot {
     std::mutex mu;
@@ -83,23 +82,16 @@ int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
     return CUSZP_SUCCESS;
 }
 
-int status_from_bits(unsigned long long bits) {
-    if (bits & kErrStall) return CUSZP_ERROR_INTERNAL;
-    if (bits & kErrOverflow) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
-    if (bits & kErrTruncated) return CUSZP_ERROR_CORRUPT_STREAM;
-    return CUSZP_SUCCESS;
-}
 
 // d_misc: [1] total of blocking calls, [2] sticky error bits (low 32 bits),
 // [4..6] watchdog diagnostics.
-ScanArgs scan_args(const Slot& s, unsigned long long* total) {
-    ScanArgs a;
-    a.status = s.d_status;
-    a.total = total ? total : s.d_misc + 1;
-    a.err = (unsigned int*)(s.d_misc + 2);
-    a.diag = s.d_misc + 4;
-    a.tag = (unsigned long long)s.epoch << kEpochShift;
-    return a;
+ScanCtx scan_ctx(const Slot& s) {
+    ScanCtx c;
+    c.status = s.d_status;
+    c.err = (unsigned int*)(s.d_misc + 2);
+    c.diag = s.d_misc + 4;
+    c.tag = (unsigned long long)s.epoch << kEpochShift;
+    return c;
 }
 
 // Read {total, error bits} after a blocking launch; clear errors once seen.
@@ -108,6 +100,7 @@ int finish_blocking(Slot& s, hipStream_t st, size_t* total) {
     CUSZP_HIP(hipStreamSynchronize(st));
     const unsigned long long bits = s.h_res[1] & 0xffffffffull;
     if (total) *total = (size_t)s.h_res[0];
+    note_steal(bits);
     if (bits & kErrStall) {
         CUSZP_HIP(hipMemcpy(s.h_res + 4, s.d_misc + 4, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         std::fprintf(stderr, "cuSZp: internal error: tile %llu stopped waiting for tile %llu (status word 0x%016llx)\n",
@@ -120,48 +113,21 @@ int finish_blocking(Slot& s, hipStream_t st, size_t* total) {
     return status_from_bits(bits);
 }
 
-// Geometry of a (dim, nbEle, dims) triple; mirrors the reference's block counts.
-int make_geom(int dim, size_t n, cuszp_dims_t d, Geom& g) {
-    std::memset(&g, 0, sizeof(g));
-    if (dim == CUSZP_DIM_1D) {
-        g.n = n;
-        g.nblocks = (n + 31) / 32;
-        g.H = (n + 32767) / 32768 * 1024;          // src/cuSZp_entry_1D_f32.cu:43
-        return CUSZP_SUCCESS;
-    }
-    if (dim != CUSZP_DIM_2D && dim != CUSZP_DIM_3D) return CUSZP_ERROR_INVALID_ARGUMENT;
-    if (!d.x || !d.y || !d.z || (d.x % 4)) return CUSZP_ERROR_INVALID_DIMS;   // float4 loads in the reference
-    const uint64_t xyz = (uint64_t)d.x * d.y * d.z;
-    if ((uint64_t)n != xyz) return CUSZP_ERROR_INVALID_DIMS;
-    g.n = n;
-    g.X = d.x; g.Y = d.y; g.Z = d.z;
-    g.XY = (uint64_t)d.x * d.y;
-    uint64_t bz;
-    if (dim == CUSZP_DIM_2D) {                      // src/cuSZp_entry_2D_f32.cu:21-23
-        g.Bx = (d.x + 7) / 8; g.By = (d.y + 7) / 8; bz = d.z;
-    } else {                                        // src/cuSZp_entry_3D_f32.cu:21-24
-        g.Bx = (d.x + 3) / 4; g.By = (d.y + 3) / 4; bz = (d.z + 3) / 4;
-    }
-    g.nblocks = (uint64_t)g.Bx * g.By * bz;
-    if (g.nblocks >= (1ull << 40)) return CUSZP_ERROR_TOO_LARGE;
-    g.H = (g.nblocks + 3) / 4 * 4;                  // src/cuSZp_kernels_2D_f32.cu:46
-    return CUSZP_SUCCESS;
-}
-
 }  // namespace
 
-int& grid_divisor(int dev) {
-    static int div[64] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
-                          1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
-                          1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
-    return div[dev & 63];
+int& grid_sixteenths(int dev) {
+    static int q[64] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
+                        16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
+                        16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
+                        16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16};
+    return q[dev & 63];
 }
 
 TilePlan plan_tiles(int dim, const Geom& g) {
     static const int forced = [] {
         const char* e = std::getenv("CUSZP_WAVES");
         const int w = e ? std::atoi(e) : 0;
-        return (w == 4 || w == 8 || w == 16) ? w : 0;
+        return (w == 4 || w == 8) ? w : 0;
     }();
     const uint64_t groups = (dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;   // 64-element units
     int waves = forced;
@@ -172,23 +138,10 @@ TilePlan plan_tiles(int dim, const Geom& g) {
 
 namespace {
 
-uint64_t max_stream_bytes(int dim, const Geom& g) {
-    const uint64_t S = (dim == CUSZP_DIM_1D) ? 4 : 8;
-    return g.H + g.nblocks * (S + 32 * S + 4);
-}
 
-bool bad_enums(int dim, int type, int mode) {
-    return (dim < 1 || dim > 3) || (type != CUSZP_TYPE_FLOAT && type != CUSZP_TYPE_DOUBLE) ||
-           (mode < CUSZP_MODE_FIXED || mode > CUSZP_MODE_OUTLIER);
-}
 
-bool bad_eb(double eb, int type) {
-    if (type == CUSZP_TYPE_FLOAT) {
-        const float f = (float)eb;
-        return !(f > 0.0f) || !std::isfinite(f);
-    }
-    return !(eb > 0.0) || !std::isfinite(eb);
-}
+
+
 
 int compress_once(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
                   unsigned long long* d_size, double eb, int dim, cuszp_dims_t dims, int type, int mode,
@@ -213,12 +166,12 @@ int compress_once(const void* in, unsigned char* out, size_t cap, size_t n, size
     std::lock_guard<std::mutex> lk(s.mu);
     rc = prepare(s, g.ntiles, st);
     if (rc) return rc;
-    const ScanArgs sa = scan_args(s, d_size);
+    const ScanCtx sc = scan_ctx(s);
+    Item item = make_item(in, out, cap, d_size ? d_size : s.d_misc + 1, (unsigned int*)(s.d_misc + 2),
+                          compress_param(eb, type), dim, type, mode, g);
     hipError_t e;
-    if (type == CUSZP_TYPE_FLOAT)
-        e = launch_compress_f32(dim, mode, plan.waves, (const float*)in, out, g, (float)eb, sa, cap, st);
-    else
-        e = launch_compress_f64(dim, mode, plan.waves, (const double*)in, out, g, eb, sa, cap, st);
+    if (type == CUSZP_TYPE_FLOAT) e = launch_compress_f32(dim, mode, plan.waves, item, sc, st);
+    else e = launch_compress_f64(dim, mode, plan.waves, item, sc, st);
     if (e != hipSuccess) return CUSZP_ERROR_HIP;
     if (!blocking) return CUSZP_SUCCESS;
     return finish_blocking(s, st, cmp_size);
@@ -242,28 +195,28 @@ int decompress_once(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
     std::lock_guard<std::mutex> lk(s.mu);
     rc = prepare(s, g.ntiles, st);
     if (rc) return rc;
-    const ScanArgs sa = scan_args(s, nullptr);
+    const ScanCtx sc = scan_ctx(s);
+    Item item = make_item(cmp, out, cmp_size, nullptr, (unsigned int*)(s.d_misc + 2), decompress_param(eb, type),
+                          dim, type, mode, g);
     hipError_t e;
-    if (type == CUSZP_TYPE_FLOAT)
-        e = launch_decompress_f32(dim, mode, plan.waves, (float*)out, cmp, g, (float)eb, sa, cmp_size, st);
-    else
-        e = launch_decompress_f64(dim, mode, plan.waves, (double*)out, cmp, g, eb, sa, cmp_size, st);
+    if (type == CUSZP_TYPE_FLOAT) e = launch_decompress_f32(dim, mode, plan.waves, item, sc, st);
+    else e = launch_decompress_f64(dim, mode, plan.waves, item, sc, st);
     if (e != hipSuccess) return CUSZP_ERROR_HIP;
     if (!blocking) return CUSZP_SUCCESS;
     return finish_blocking(s, st, nullptr);
 }
 
-// A stalled scan means a persistent grid larger than what is resident: halve
-// every persistent grid on this device and run the call again (the inputs are
-// untouched by a stalled launch).
+// A stalled scan (the watchdog gave up; not expected while predecessors can be
+// recomputed by waiting waves) is retried once on a quarter-size grid (the
+// inputs are untouched by a stalled launch).
 bool shrink_after_stall(int rc) {
     if (rc != CUSZP_ERROR_INTERNAL) return false;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
-    int& d = grid_divisor(dev);
-    if (d >= 64) return false;
-    d *= 2;
-    std::fprintf(stderr, "cuSZp: persistent grid reduced to 1/%d of the occupancy estimate\n", d);
+    int& q = grid_sixteenths(dev);
+    if (q <= 4) return false;
+    q = 4;
+    std::fprintf(stderr, "cuSZp: persistent grid reduced to a quarter of the occupancy estimate\n");
     return true;
 }
 

@@ -10,11 +10,10 @@
 //     blocks in 1D, one 8x8 (2D) or 4x4x4 (3D) block otherwise. One wave holds
 //     one group, one element per lane;
 //   * a workgroup is WAVES waves; each wave owns G consecutive groups, so a
-//     tile is WAVES*G groups. The grid is persistent and cooperatively
-//     launched (every workgroup resident): workgroup w handles tiles w, w+B,
-//     w+2B, ... and prefetches its next tile while it scans and stores the
-//     current one. No ticket atomics: one hot device-scope atomic serialises
-//     at about 12 ns per operation on MI355X;
+//     tile is WAVES*G groups. The grid is persistent: workgroup w handles
+//     tiles w, w+B, w+2B, ... and prefetches its next tile while it scans and
+//     stores the current one. No ticket atomics: one hot device-scope atomic
+//     serialises at about 12 ns per operation on MI355X;
 //   * lanes hold elements in "mirror" order: lane L holds element L^7, which is
 //     exactly the bit order of cuSZp's sign/plane bytes (element e at byte e/8,
 //     bit 7-e%8). Each lane packs its sign into bit 31 above the 31-bit
@@ -26,10 +25,13 @@
 //     wave with one lane per block;
 //   * the inter-tile exclusive scan of compressed bytes is a single-pass
 //     decoupled lookback: one 64-bit status word per tile packs
-//     {epoch, flag, value}, is read 64 predecessors at a time by one wave, and
+//     {epoch, flag, value}, is read 256 predecessors at a time by one wave, and
 //     needs only relaxed agent-scope atomics (no fences: the value travels in
-//     the same word as its flag). With all workgroups resident, the lowest
-//     unfinished tile can always complete, so the walk always terminates.
+//     the same word as its flag). Forward progress does not rely on every
+//     workgroup being resident: a wave that has waited too long for a
+//     predecessor claims it and recomputes that tile's byte count itself;
+//   * the batched kernels run the same per-variant loops over a table of
+//     arrays; each array's tiles form their own scan segment.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,46 +46,77 @@ constexpr int kWaveSize = 64;
 enum : int { kFixed = 0, kPlain = 1, kOutlier = 2 };
 
 // Tile status word: [63:40] epoch tag, [39:38] flag, [37:0] byte count.
+// Flags: 1 = aggregate of the tile, 2 = inclusive prefix, 3 = claimed by a
+// waiting wave that is computing the aggregate; 0 = nothing yet.
 constexpr int kEpochShift = 40;
+constexpr uint64_t kEpochMask = (1ull << 24) - 1;
 constexpr uint64_t kFlagAgg = 1ull << 38;
 constexpr uint64_t kFlagPrefix = 2ull << 38;
+constexpr uint64_t kFlagClaim = 3ull << 38;
 constexpr uint64_t kFlagMask = 3ull << 38;
 constexpr uint64_t kValueMask = (1ull << 38) - 1;
+constexpr uint64_t kTagMask = ~(kFlagMask | kValueMask);
 
-// Error bits reported through ScanArgs::err.
+// Bits of the sticky error words.
 constexpr uint32_t kErrOverflow = 1u;    // compressed stream exceeds output capacity
 constexpr uint32_t kErrTruncated = 2u;   // stream shorter than its header describes
 constexpr uint32_t kErrStall = 4u;       // a lookback gave up waiting (internal error; output invalid)
+constexpr uint32_t kInfoStolen = 8u;     // a waiting wave computed a predecessor's byte count (not an error)
+constexpr uint32_t kErrMask = 7u;
 
-// Bound on lookback polls (each polls the status words once and sleeps ~64
-// cycles): far above any legitimate wait, it guarantees that every wave of a
-// launch exits even if a predecessor tile never publishes.
+// Bound on lookback polls (each polls the status words once): far above any
+// legitimate wait, it guarantees that every wave of a launch exits even if a
+// predecessor tile never publishes.
 constexpr uint32_t kSpinLimit = 1u << 18;
+// A wait longer than this (s_memrealtime ticks: 100 us at the 100 MHz
+// reference clock) makes the waiting wave compute pending predecessors itself.
+constexpr uint64_t kStealTicks = 10000;
 
 struct Geom {
     uint64_t n;         // elements
     uint64_t nblocks;   // cuSZp blocks
     uint64_t H;         // header bytes
-    uint64_t ntiles;    // workgroups in the launch
+    uint64_t ntiles;    // tiles of the array
     uint64_t XY;        // X*Y
     uint32_t X, Y, Z;
     uint32_t Bx, By;    // blocks along x and y (2D/3D)
 };
 
-struct ScanArgs {
-    unsigned long long* status;     // [ntiles] tile status words
-    unsigned long long* total;      // out: total stream bytes (written by the last tile), may be null
-    unsigned int* err;              // sticky error bits (kErr*)
-    unsigned long long* diag;       // [0] first stalled tile + 1, [1] its missing predecessor, [2] tag seen
+// Launch-wide scan state.
+struct ScanCtx {
+    unsigned long long* status;     // tile status words, one per tile of the launch
+    unsigned int* err;              // sticky error bits of the launch (kErrStall, kInfoStolen)
+    unsigned long long* diag;       // [0] first stalled tile + 1, [1] its missing predecessor, [2] word seen
     unsigned long long tag;         // epoch << kEpochShift
+};
+
+// One array of a launch. The single-array kernels take it as an argument; the
+// batched kernels read a table of them.
+struct Item {
+    const void* src;                // compress: values; decompress: stream
+    void* dst;                      // compress: stream; decompress: values
+    uint64_t limit;                 // compress: stream capacity; decompress: stream length
+    unsigned long long* total;      // out: stream length, written by the last tile (may be null)
+    unsigned int* err;              // sticky error bits of this array (kErrOverflow, kErrTruncated)
+    double param;                   // compress: 0.5/eb, decompress: eb (exact in the array's type)
+    uint64_t tile0;                 // first tile of the array in the launch (its scan segment start)
+    uint32_t variant;               // type * 9 + (dim - 1) * 3 + mode
+    uint32_t pad_;
+    Geom geo;                       // geo.ntiles: tiles of this array
 };
 
 // ---------------------------------------------------------------------------
 // wave64 primitives
 // ---------------------------------------------------------------------------
+// Lanes whose source is outside the row/wave read 0. With every row and bank
+// enabled that is bound_ctrl (no old value to materialise); masked rows/banks
+// keep the old value 0.
 template <int CTRL, int ROWS = 0xf, int BANKS = 0xf>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, BANKS, false);
+    if constexpr (ROWS == 0xf && BANKS == 0xf)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, ROWS, BANKS, true);
+    else
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, BANKS, false);
 }
 constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
 constexpr int kDppWaveShr1 = 0x138, kDppRowMirror = 0x140, kDppHalfMirror = 0x141;
@@ -129,7 +162,12 @@ __device__ __forceinline__ uint32_t swizzle_xor(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1f | (J << 10));
 }
 __device__ __forceinline__ uint32_t rotr(uint32_t v, uint32_t s) { return __builtin_amdgcn_alignbit(v, v, s); }
-__device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) { return (m & a) | (~m & b); }
+// (m & a) | (~m & b) in one v_bfi_b32 (the compiler otherwise emits and + and_or)
+__device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+    return r;
+}
 __device__ __forceinline__ uint32_t xpose32(uint32_t x, const XposeConsts& c) {
     x = __builtin_amdgcn_perm(swizzle_xor<16>(x), x, c.sel16);
     x = __builtin_amdgcn_perm(swizzle_xor<8>(x), x, c.sel8);
@@ -247,13 +285,22 @@ __device__ __forceinline__ uint32_t load_u32u(const uint8_t* p) {
 }
 
 // ---------------------------------------------------------------------------
-// decoupled lookback (single pass, wave-parallel)
+// decoupled lookback (single pass, wave-parallel, segmented, stealing)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t ld_status(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_status(unsigned long long* p, uint64_t v) {
     __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t cas_status(unsigned long long* p, uint64_t expect, uint64_t desired) {
+    unsigned long long e = expect;
+    __hip_atomic_compare_exchange_strong(p, &e, (unsigned long long)desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return e;   // value seen (== expect on success)
+}
+__device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
+    return (uint64_t)rdlane((uint32_t)v, l) | ((uint64_t)rdlane((uint32_t)(v >> 32), l) << 32);
 }
 
 // Sum over the wave of per-lane values below 2^40: two 20-bit halves are
@@ -264,117 +311,182 @@ __device__ __forceinline__ uint64_t wave_sum_u40(uint64_t v) {
     return (uint64_t)rdlane(lo, kWaveSize - 1) + ((uint64_t)rdlane(hi, kWaveSize - 1) << 20);
 }
 
-constexpr int kLookbackWords = 4;   // predecessors read per lane and round trip: 256 per wave
-
 __device__ __forceinline__ bool status_ready(uint64_t w, uint64_t tag) {
-    return ((w & ~(kFlagMask | kValueMask)) == tag) && (w & kFlagMask);
+    const uint64_t f = w & kFlagMask;
+    return ((w & kTagMask) == tag) && f != 0 && f != kFlagClaim;
 }
 
-// Called by one whole wave. Publishes the tile aggregate, walks back over
-// predecessors 256 at a time (position p = lane + 64*k is predecessor
-// tile - 1 - p), publishes the inclusive prefix and returns the exclusive
-// prefix (same value in every lane). One round trip advances the walk by up to
-// 256 tiles, so the prefix front keeps up with tiles issued 256x faster than a
-// status round trip.
-__device__ __forceinline__ uint64_t tile_lookback(const ScanArgs& sa, uint64_t tile, uint64_t agg, int lane) {
-    const uint64_t tag = sa.tag;
-    if (tile == 0) {
-        if (lane == 0) st_status(sa.status, tag | kFlagPrefix | agg);
+// Lookback state shared by the waves of a workgroup.
+template <int WAVES>
+struct LookbackShared {
+    uint32_t pos[2][WAVES][2];   // per poll round (double-buffered): nearest prefix, nearest pending position
+    uint32_t ctl[2];             // wave 0's decisions per poll round: bit 0 steal, bit 1 give up
+    uint64_t part[WAVES];        // per-wave sums of a resolved window
+};
+
+// Exclusive prefix of `tile` within its segment (tiles first .. tile-1), run by
+// every wave of the workgroup (uniform control flow across the workgroup).
+//
+// The workgroup publishes the tile aggregate, then reads predecessors a window
+// of WAVES * 64 * K at a time: wave v, lane j, word k reads position
+// p = 64*(K*v + k) + j, i.e. tile - 1 - p. A window is resolved once every
+// position up to the nearest published inclusive prefix P is ready; the sum of
+// the values at positions <= P is the exclusive prefix (or, with no prefix in
+// the window, its sum carries into the next window). The inclusive prefix is
+// then published. In a persistent grid of B workgroups the nearest prefix
+// trails a tile by up to about B positions, so the window is sized to cover B
+// in one round trip of status loads.
+//
+// Forward progress does not depend on which workgroups are resident: when a
+// window stays unresolved for kStealTicks, each wave claims its farthest
+// pending position (CAS from the stale word to CLAIM), computes that tile's
+// aggregate with agg_of(tile) and publishes it. A claim is held only by a
+// running wave that computes without waiting, so every wait ends; the owner's
+// later store of the same aggregate (or of its prefix) is harmless.
+template <int WAVES, int K, class AggF>
+__device__ __forceinline__ uint64_t wg_lookback(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t agg,
+                                                int wave, int lane, LookbackShared<WAVES>& ls, AggF&& agg_of) {
+    constexpr int SPAN = 64 * K;           // positions per wave and window
+    constexpr int WIN = WAVES * SPAN;      // positions per window
+    const uint64_t tag = sc.tag;
+    const bool leader = (wave == 0) && (lane == 0);
+    if (tile == first) {
+        if (leader) st_status(sc.status + tile, tag | kFlagPrefix | agg);
         return 0;
     }
-    if (lane == 0) st_status(sa.status + tile, tag | kFlagAgg | agg);
+    if (leader) st_status(sc.status + tile, tag | kFlagAgg | agg);
+    const int pbase = wave * SPAN;
     uint64_t excl = 0;
     int64_t hi = (int64_t)tile;
-    uint32_t spins = 0;
-    bool stalled = false;
+    uint32_t spins = 0, round = 0;
+    const uint64_t t_wait = __builtin_amdgcn_s_memrealtime();
+    bool steal = false, stalled = false;
     for (;;) {
-        uint64_t w[kLookbackWords];
-        bool ready[kLookbackWords];
+        uint64_t w[K];
+        bool ready[K];
 #pragma unroll
-        for (int k = 0; k < kLookbackWords; ++k) {
-            const int64_t idx = hi - 1 - lane - 64 * k;
-            w[k] = tag | kFlagPrefix;   // before tile 0: inclusive prefix 0
+        for (int k = 0; k < K; ++k) {
+            const int64_t idx = hi - 1 - (pbase + 64 * k + lane);
+            w[k] = tag | kFlagPrefix;   // before the segment: inclusive prefix 0
             ready[k] = true;
-            if (idx >= 0) {
-                w[k] = ld_status(sa.status + idx);
+            if (idx >= (int64_t)first) {
+                w[k] = ld_status(sc.status + idx);
                 ready[k] = status_ready(w[k], tag);
             }
         }
-        // nearest published prefix P (position); wait until positions <= P are ready
         int P;
         for (;;) {
-            P = kLookbackWords * 64;
-            bool all_ready = true;
+            int Pv = WIN, Nv = WIN;
 #pragma unroll
-            for (int k = kLookbackWords - 1; k >= 0; --k) {
+            for (int k = K - 1; k >= 0; --k) {
                 const uint64_t pm = ballot(ready[k] && (w[k] & kFlagMask) == kFlagPrefix);
-                if (pm) P = 64 * k + __builtin_ctzll(pm);
-            }
-#pragma unroll
-            for (int k = 0; k < kLookbackWords; ++k) {
+                if (pm) Pv = pbase + 64 * k + __builtin_ctzll(pm);
                 const uint64_t nr = ballot(!ready[k]);
-                const int lo = 64 * k;
-                if (P >= lo + 63) {
-                    all_ready &= (nr == 0);
-                } else if (P >= lo) {
-                    all_ready &= ((nr & ((2ull << (P - lo)) - 1)) == 0);
+                if (nr) Nv = pbase + 64 * k + __builtin_ctzll(nr);
+            }
+            const uint32_t par = round & 1;
+            if (lane == 0) {
+                ls.pos[par][wave][0] = (uint32_t)Pv;
+                ls.pos[par][wave][1] = (uint32_t)Nv;
+                if (wave == 0) {
+                    uint32_t c = 0;
+                    if (__builtin_amdgcn_s_memrealtime() - t_wait > kStealTicks) c |= 1;
+                    if (spins + 1 >= kSpinLimit ||
+                        (((spins + 1) & 1023) == 0 &&
+                         (__hip_atomic_load(sc.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kErrStall)))
+                        c |= 2;
+                    ls.ctl[par] = c;
                 }
             }
-            if (all_ready) break;
-            if (++spins >= kSpinLimit ||
-                ((spins & 1023) == 0 && (__hip_atomic_load(sa.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kErrStall))) {
-                // watchdog: record the first stuck predecessor and give up (the
-                // stream is reported invalid; publishing below unblocks successors)
-                stalled = true;
-                uint64_t first = ~0ull, seen = 0;
+            __syncthreads();
+            P = WIN;
+            int N = WIN;
 #pragma unroll
-                for (int k = 0; k < kLookbackWords; ++k) {
-                    const uint64_t nr = ballot(!ready[k]);
-                    if (nr && first == ~0ull) {
-                        const int ln = __builtin_ctzll(nr);
-                        first = (uint64_t)(hi - 1 - ln - 64 * k);
-                        seen = __builtin_amdgcn_readlane((unsigned)w[k], ln) | ((uint64_t)__builtin_amdgcn_readlane((unsigned)(w[k] >> 32), ln) << 32);
-                    }
-                }
-                if (lane == 0) {
-                    atomicOr(sa.err, kErrStall);
-                    if (atomicCAS(sa.diag, 0ull, tile + 1) == 0ull) {
-                        sa.diag[1] = first;
-                        sa.diag[2] = seen;
+            for (int v = 0; v < WAVES; ++v) {
+                P = min(P, (int)ls.pos[par][v][0]);
+                N = min(N, (int)ls.pos[par][v][1]);
+            }
+            const uint32_t ctl = ls.ctl[par];
+            ++round;
+            ++spins;
+            if (N > P || N == WIN) break;   // every position up to P is ready
+            // nearest pending position of this wave at or below P
+            int sk = -1, sl = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                uint64_t nr = ballot(!ready[k]);
+                const int lo = pbase + 64 * k;
+                if (P < lo) nr = 0;
+                else if (P < lo + 63) nr &= (2ull << (P - lo)) - 1;
+                if (nr) { sk = k; sl = 63 - __builtin_clzll(nr); }
+            }
+            uint64_t pend = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) if (k == sk) pend = rdlane64(w[k], sl);
+            const uint64_t pidx = (uint64_t)(hi - 1 - (pbase + 64 * sk + sl));
+            if (ctl & 2) {
+                // last-resort watchdog: record the stuck predecessor and give up
+                // (the output is reported invalid; publishing below unblocks successors)
+                stalled = true;
+                if (lane == 0 && sk >= 0) {
+                    atomicOr(sc.err, kErrStall);
+                    if (atomicCAS(sc.diag, 0ull, tile + 1) == 0ull) {
+                        sc.diag[1] = pidx;
+                        sc.diag[2] = pend;
                     }
                 }
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            steal |= (ctl & 1) != 0;
+            bool took = false;
+            if (steal && sk >= 0 && !(((pend & kTagMask) == tag) && ((pend & kFlagMask) == kFlagClaim))) {
+                uint64_t seen = 0;
+                if (lane == 0) seen = cas_status(sc.status + pidx, pend, tag | kFlagClaim);
+                seen = rdlane64(seen, 0);
+                if (seen == pend) {
+                    const uint64_t a = agg_of(pidx);
+                    if (lane == 0) {
+                        cas_status(sc.status + pidx, tag | kFlagClaim, tag | kFlagAgg | a);
+                        atomicOr(sc.err, kInfoStolen);
+                    }
+                    took = true;
+                }
+            }
+            if (!took) __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-            for (int k = 0; k < kLookbackWords; ++k) {
-                const int64_t idx = hi - 1 - lane - 64 * k;
+            for (int k = 0; k < K; ++k) {
+                const int64_t idx = hi - 1 - (pbase + 64 * k + lane);
                 if (!ready[k]) {
-                    w[k] = ld_status(sa.status + idx);
+                    w[k] = ld_status(sc.status + idx);
                     ready[k] = status_ready(w[k], tag);
                 }
             }
         }
         uint64_t part = 0;
 #pragma unroll
-        for (int k = 0; k < kLookbackWords; ++k) part += (lane + 64 * k <= P) ? (w[k] & kValueMask) : 0;
-        excl += wave_sum_u40(part);
-        if (P < kLookbackWords * 64 || stalled) break;
-        hi -= kLookbackWords * 64;
+        for (int k = 0; k < K; ++k) part += (pbase + 64 * k + lane <= P) ? (w[k] & kValueMask) : 0;
+        part = wave_sum_u40(part);
+        if (lane == 0) ls.part[wave] = part;
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < WAVES; ++v) excl += ls.part[v];
+        if (P < WIN || stalled) break;
+        hi -= WIN;
     }
-    if (lane == 0) st_status(sa.status + tile, tag | kFlagPrefix | (excl + agg));
+    if (leader) st_status(sc.status + tile, tag | kFlagPrefix | (excl + agg));
     return excl;
 }
 
-// Workgroup-level: combine the wave totals, run the lookback on wave 0 and
-// return {wave's exclusive offset within the stream payload, tile inclusive}.
+// Workgroup-level: combine the wave totals and run the lookback; returns
+// {wave's exclusive offset within the stream payload, tile inclusive}.
 struct TileOffsets {
     uint64_t wave_base;   // payload offset (excluding H) of this wave's first block
     uint64_t tile_incl;   // payload bytes up to and including this tile
 };
-template <int WAVES>
-__device__ __forceinline__ TileOffsets tile_scan(const ScanArgs& sa, uint64_t tile, uint64_t wtot,
-                                                 int wave, int lane, uint64_t* s_wtot, uint64_t* s_excl) {
+template <int WAVES, int K, class AggF>
+__device__ __forceinline__ TileOffsets tile_scan(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t wtot,
+                                                 int wave, int lane, uint64_t* s_wtot, LookbackShared<WAVES>& ls,
+                                                 AggF&& agg_of) {
     if (lane == 0) s_wtot[wave] = wtot;
     __syncthreads();
     uint64_t wexcl = 0, ttot = 0;
@@ -384,14 +496,10 @@ __device__ __forceinline__ TileOffsets tile_scan(const ScanArgs& sa, uint64_t ti
         wexcl += (w < wave) ? v : 0;
         ttot += v;
     }
-    if (wave == 0) {
-        const uint64_t excl = tile_lookback(sa, tile, ttot, lane);
-        if (lane == 0) *s_excl = excl;
-    }
-    __syncthreads();
-    const uint64_t excl = *s_excl;
+    const uint64_t excl = wg_lookback<WAVES, K>(sc, tile, first, ttot, wave, lane, ls, agg_of);
     return TileOffsets{excl + wexcl, excl + ttot};
 }
+
 
 // ---------------------------------------------------------------------------
 // Block walking for 2D/3D: wave-uniform block coordinates advanced by +1 block.
@@ -539,66 +647,76 @@ __device__ __forceinline__ uint32_t word_offset(uint32_t l, uint32_t h, bool sig
 }
 
 // ---------------------------------------------------------------------------
-// compress
+// Shared memory of one workgroup, sized for the largest variant of a launch.
 // ---------------------------------------------------------------------------
+template <int WAVES, int GMAX>
+struct Smem {
+    LookbackShared<WAVES> lb;
+    uint64_t wtot[WAVES];
+    uint32_t meta[WAVES][2 * GMAX];   // per-block metadata (pack_meta), one entry per block of a wave
+    uint32_t aux[WAVES][2 * GMAX];    // compress: sign words of wide 2D/3D groups; decompress: outlier |d0|
+};
+
+// Item sources: where a persistent loop finds the array a tile belongs to.
+// OneItem serves the single-array kernels (the descriptor is a kernel
+// argument); ItemTable serves the batched kernels (descriptors in device
+// memory, ordered by tile0).
+struct OneItem {
+    Item item;
+    __device__ __forceinline__ uint32_t locate(uint64_t, uint32_t) const { return 0; }
+    __device__ __forceinline__ Item get(uint32_t) const { return item; }
+    __device__ __forceinline__ uint32_t variant(uint32_t) const { return item.variant; }
+};
+struct ItemTable {
+    const Item* items;
+    uint32_t count;
+    // last item whose tile0 <= tile, searching from `from` (tiles only move forward)
+    __device__ __forceinline__ uint32_t locate(uint64_t tile, uint32_t from) const {
+        uint32_t lo = from, hi = count;   // answer in [lo, hi)
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (items[mid].tile0 <= tile) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    }
+    __device__ __forceinline__ Item get(uint32_t i) const { return items[i]; }
+    __device__ __forceinline__ uint32_t variant(uint32_t i) const { return items[i].variant; }
+};
+
+// ---------------------------------------------------------------------------
+// Aggregates recomputed by a waiting wave (see tile_lookback).
+// ---------------------------------------------------------------------------
+// OR of v over each 32-lane half (every lane of the half gets the result).
+__device__ __forceinline__ uint32_t or_half(uint32_t v) {
+    v |= swizzle_xor<16>(v);
+    v |= swizzle_xor<8>(v);
+    v |= swizzle_xor<4>(v);
+    v |= swizzle_xor<2>(v);
+    v |= swizzle_xor<1>(v);
+    return v;
+}
+
+// Payload bytes of local tile `lt` of a compress item, by one wave. Same block
+// sizes as pass 1 of compress_run: a block's rate is the bit length of the OR of
+// its magnitudes (the highest non-zero plane), 1D ranks |-2^31| as 0.
 template <int DIM, int MODE, class T, int G, int WAVES>
-__global__ void __launch_bounds__(WAVES * kWaveSize)
-compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom geo, const T r,
-                const ScanArgs sa, const uint64_t cap) {
-    constexpr int S = (DIM == 1) ? 4 : 8;        // bytes per plane
-    constexpr int BPG = (DIM == 1) ? 2 : 1;      // blocks per group
-    constexpr int NB = BPG * G;                  // blocks per wave
-    static_assert(NB <= kWaveSize, "one bookkeeping lane per block");
-    __shared__ uint64_t s_excl;
-    __shared__ uint64_t s_wtot[WAVES];
-    __shared__ uint32_t s_meta[WAVES][NB];
-    __shared__ uint32_t s_sgn[WAVES][G][2];     // sign words of 2D/3D blocks holding |d| = 2^31
-
-    const int lane = (int)__lane_id();
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
-
+__device__ uint64_t compress_tile_bytes(const T* __restrict__ in, const Geom& geo, T r, uint64_t lt, int lane) {
+    constexpr int S = (DIM == 1) ? 4 : 8;
     LaneGeo<DIM> lg;
     lg.init(lane, geo);
     const int pl = pred_lane<DIM>(lane);
     const int pl_src = pl < 0 ? lane : pl;
-    XposeConsts xc;
-    xc.init(lane);
-    // after the transpose, element 0 of a block is bit 7 of its plane words: in
-    // both halves for 1D (two blocks), only in the low half for 2D/3D
-    const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
-
-    // All loads of a tile's G groups per wave are issued back to back; the next
-    // tile's loads are issued before the current tile's scan and stores.
-    T xv[G];
-    uint64_t vmask = 0;   // bit g: this lane's element of group g lies inside the array
-    auto load_tile = [&](uint64_t tl) {
-        const uint64_t g0 = (tl * WAVES + (uint64_t)wave) * G;
-        Walker wk;
-        if constexpr (DIM != 1) wk.init(g0, geo);
-        vmask = 0;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            uint64_t idx;
-            const bool valid = elem_of<DIM>(geo, lg, wk, g0 + g, lane, idx);
-            xv[g] = valid ? __builtin_nontemporal_load(in + idx) : T(0);
-            vmask |= (uint64_t)valid << g;
-            if constexpr (DIM != 1) wk.next(geo);
-        }
-    };
-    const uint64_t stride = gridDim.x;
-    uint64_t tile = blockIdx.x;
-    if (tile < geo.ntiles) load_tile(tile);
-    for (; tile < geo.ntiles; tile += stride) {
-    const uint64_t grp0 = (tile * WAVES + (uint64_t)wave) * G;
-
-    uint32_t W[G];
-    uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
-    uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
-    static_for<0, G>([&](auto gc) {
-        constexpr int g = decltype(gc)::value;
-        const bool valid = (vmask >> g) & 1;
-        const uint32_t q = quantize<T>(xv[g], r);   // x = 0 outside the array gives q = 0
+    const bool e0 = (DIM == 1) ? ((lane & 31) == 0) : (lane == 0);
+    const uint64_t g0 = lt * WAVES * G;
+    Walker wk;
+    if constexpr (DIM != 1) wk.init(g0, geo);
+    uint64_t sum = 0;
+    for (int i = 0; i < WAVES * G; ++i) {
+        uint64_t idx;
+        const bool valid = elem_of<DIM>(geo, lg, wk, g0 + i, lane, idx);
+        const T x = valid ? in[idx] : T(0);
+        const uint32_t q = quantize<T>(x, r);
         uint32_t d;
         if constexpr (MODE == kFixed) {
             d = q;
@@ -612,6 +730,126 @@ compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom 
             d = q - qp;
         }
         d = valid ? d : 0u;
+        const uint32_t a = ((int32_t)d < 0) ? 0u - d : d;
+        const uint32_t a1 = (DIM == 1) ? (a & 0x7fffffffu) : a;
+        const uint32_t o1 = or_half(a1);
+        const uint32_t o2 = (MODE == kOutlier) ? or_half(e0 ? 0u : a1) : 0u;
+        if constexpr (DIM == 1) {
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const uint32_t byte = rate_byte<MODE, S>(bitlen(rdlane(o1, 32 * hh)), bitlen(rdlane(o2, 32 * hh)),
+                                                         bitlen(rdlane(a, 32 * hh)));
+                sum += view_of<MODE, S>(byte).size;
+            }
+        } else {
+            const uint32_t byte = rate_byte<MODE, S>(bitlen(rdlane(o1, 0) | rdlane(o1, 32)),
+                                                     bitlen(rdlane(o2, 0) | rdlane(o2, 32)), bitlen(rdlane(a, 0)));
+            sum += view_of<MODE, S>(byte).size;
+            wk.next(geo);
+        }
+    }
+    return sum;
+}
+
+// Payload bytes of local tile `lt` of a decompress item, from its header bytes.
+template <int DIM, int MODE, int G, int WAVES>
+__device__ uint64_t header_tile_bytes(const uint8_t* __restrict__ cmp, const Geom& geo, uint64_t lt, int lane) {
+    constexpr int S = (DIM == 1) ? 4 : 8;
+    constexpr int BPT = WAVES * G * ((DIM == 1) ? 2 : 1);   // blocks per tile
+    const uint64_t limit = (DIM == 1) ? geo.H : geo.nblocks;
+    uint64_t sum = 0;
+#pragma unroll
+    for (int c = 0; c < (BPT + kWaveSize - 1) / kWaveSize; ++c) {
+        const uint64_t b = lt * BPT + (uint64_t)(c * kWaveSize + lane);
+        const bool in_tile = c * kWaveSize + lane < BPT;
+        const uint32_t byte = (in_tile && b < limit) ? (uint32_t)cmp[b] : 0u;
+        sum += wave_sum_u40(view_of<MODE, S>(byte).size);
+    }
+    return sum;
+}
+
+// ---------------------------------------------------------------------------
+// compress: persistent loop over the tiles of consecutive items of one variant
+// ---------------------------------------------------------------------------
+// Processes tiles tile, tile + gridDim.x, ... while they belong to items of this
+// variant; returns with `tile` at the first tile it did not process and `it` at
+// that tile's item (or tile >= ntiles).
+template <int DIM, int MODE, class T, int G, int WAVES, int GMAX, class Src>
+__device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint64_t& tile, const uint64_t ntiles,
+                                             const ScanCtx& sc, Smem<WAVES, GMAX>& sm) {
+    constexpr int S = (DIM == 1) ? 4 : 8;        // bytes per plane
+    constexpr int BPG = (DIM == 1) ? 2 : 1;      // blocks per group
+    constexpr int NB = BPG * G;                  // blocks per wave
+    static_assert(NB <= kWaveSize, "one bookkeeping lane per block");
+    static_assert(G <= GMAX, "shared memory sized for the largest variant");
+
+    const int lane = (int)__lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
+    const int pl = pred_lane<DIM>(lane);
+    const int pl_src = pl < 0 ? lane : pl;
+    XposeConsts xc;
+    xc.init(lane);
+    // after the transpose, element 0 of a block is bit 7 of its plane words: in
+    // both halves for 1D (two blocks), only in the low half for 2D/3D
+    const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
+    const uint64_t stride = gridDim.x;
+
+    Item cur = src.get(it);
+    LaneGeo<DIM> lg;
+    lg.init(lane, cur.geo);
+
+    // All loads of a tile's G groups per wave are issued back to back; the next
+    // tile's loads (same item) are issued before the current tile's scan and stores.
+    T xv[G];
+    uint64_t vmask = 0;   // bit g: this lane's element of group g lies inside the array
+    auto load_tile = [&](uint64_t lt) {
+        const Geom& geo = cur.geo;
+        const T* __restrict__ in = (const T*)cur.src;
+        const uint64_t g0 = (lt * WAVES + (uint64_t)wave) * G;
+        Walker wk;
+        if constexpr (DIM != 1) wk.init(g0, geo);
+        vmask = 0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            uint64_t idx;
+            const bool valid = elem_of<DIM>(geo, lg, wk, g0 + g, lane, idx);
+            xv[g] = valid ? __builtin_nontemporal_load(in + idx) : T(0);
+            vmask |= (uint64_t)valid << g;
+            if constexpr (DIM != 1) wk.next(geo);
+        }
+    };
+    load_tile(tile - cur.tile0);
+
+    for (;;) {
+    const Geom& geo = cur.geo;
+    const uint64_t lt = tile - cur.tile0;
+    const T r = (T)cur.param;
+    const uint64_t grp0 = (lt * WAVES + (uint64_t)wave) * G;
+
+    uint32_t W[G];
+    uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
+    uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
+    // Pass 1 has two instantiations: tiles entirely inside the array skip the
+    // per-element validity select (fixed mode never needs it: x = 0 gives q = 0).
+    auto pass1 = [&](auto fullc) {
+    constexpr bool FULL = decltype(fullc)::value;
+    static_for<0, G>([&](auto gc) {
+        constexpr int g = decltype(gc)::value;
+        const uint32_t q = quantize<T>(xv[g], r);   // x = 0 outside the array gives q = 0
+        uint32_t d;
+        if constexpr (MODE == kFixed) {
+            d = q;
+        } else if constexpr (DIM == 1) {
+            uint32_t qp = dpp<kDppWaveShr1>(q);
+            qp = (lane & 31) ? qp : 0u;
+            d = q - qp;
+        } else {
+            uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
+            qp = (pl >= 0) ? qp : 0u;
+            d = q - qp;
+        }
+        if constexpr (!FULL && MODE != kFixed) d = ((vmask >> g) & 1) ? d : 0u;
         const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
         const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
         uint64_t m1, m2 = 0;
@@ -631,8 +869,8 @@ compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom 
             if constexpr (MODE == kOutlier) m2 = ballot((w & not_e0) != 0);
             const uint64_t sg = ballot((int32_t)dm < 0);
             if (lane == 0) {
-                s_sgn[wave][g][0] = (uint32_t)sg;
-                s_sgn[wave][g][1] = (uint32_t)(sg >> 32);
+                sm.aux[wave][2 * g] = (uint32_t)sg;
+                sm.aux[wave][2 * g + 1] = (uint32_t)(sg >> 32);
             }
             big |= 1u << g;
         }
@@ -653,10 +891,16 @@ compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom 
             }
         }
     });
+    };
+    if (ballot(vmask != (G == 64 ? ~0ull : (1ull << G) - 1)) == 0) pass1(std::true_type{});
+    else pass1(std::false_type{});
 
-    if (tile + stride < geo.ntiles) load_tile(tile + stride);   // in flight during scan + stores
+    const uint64_t next = tile + stride;
+    const bool same = next < cur.tile0 + geo.ntiles;   // next tile in this item: prefetch it now
+    if (same) load_tile(next - cur.tile0);             // in flight during scan + stores
 
     // ---- bookkeeping: lane j <-> block j of this wave -----------------------
+    uint8_t* __restrict__ out = (uint8_t*)cur.dst;
     const uint64_t blk_idx = (DIM == 1) ? 2 * grp0 + (uint64_t)lane : grp0 + (uint64_t)lane;
     const bool blk = lane < NB;
     uint32_t byte;
@@ -668,65 +912,77 @@ compress_kernel(const T* __restrict__ in, uint8_t* __restrict__ out, const Geom 
     const uint32_t off = incl - sz;
     const uint64_t wtot = rdlane(incl, kWaveSize - 1);
     if (blk) {
-        s_meta[wave][lane] = pack_meta(off, v.R, sz != 0, v.sel, v.head);
+        sm.meta[wave][lane] = pack_meta(off, v.R, sz != 0, v.sel, v.head);
         // header byte; 2D/3D pad bytes [nblocks, H) are written as 0
         if (DIM == 1 || blk_idx < geo.H) out[blk_idx] = (uint8_t)byte;
     }
 
-    const TileOffsets to = tile_scan<WAVES>(sa, tile, wtot, wave, lane, s_wtot, &s_excl);
-    if (tile == geo.ntiles - 1 && threadIdx.x == 0 && sa.total) *sa.total = geo.H + to.tile_incl;
+    const TileOffsets to = tile_scan<WAVES, (8 / WAVES > 1 ? 8 / WAVES : 1)>(sc, tile, cur.tile0, wtot, wave, lane, sm.wtot, sm.lb,
+        [&](uint64_t p) { return compress_tile_bytes<DIM, MODE, T, G, WAVES>((const T*)cur.src, geo, r, p - cur.tile0, lane); });
+    if (lt == geo.ntiles - 1 && threadIdx.x == 0 && cur.total) *cur.total = geo.H + to.tile_incl;
     uint8_t* const wbase = out + geo.H + to.wave_base;
-    if (geo.H + to.wave_base + wtot > cap) {   // never write past the caller's buffer
-        if (lane == 0) atomicOr(sa.err, kErrOverflow);
-        continue;
-    }
-    if constexpr (MODE == kOutlier) {
-        if (blk && v.sel) store_le(wbase + off, bko, v.nb);   // little-endian |d| of element 0
+    const bool fits = geo.H + to.wave_base + wtot <= cur.limit;   // never write past the caller's buffer
+    if (!fits && lane == 0) atomicOr(cur.err, kErrOverflow);
+    if (fits) {
+        if constexpr (MODE == kOutlier) {
+            if (blk && v.sel) store_le(wbase + off, bko, v.nb);   // little-endian |d| of element 0
+        }
+        // ---- pass 2: stream words --------------------------------------------
+        static_for<0, G>([&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            const Meta m(sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
+            const bool wideg = (DIM != 1) && ((big >> g) & 1);
+            uint32_t w = W[g];
+            if constexpr (MODE == kOutlier) {
+                if (m.sel && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
+            }
+            const bool st = m.nz && (l < m.R || (l == 31 && !wideg));
+            if (st) store_u32u(wbase + m.off + m.head + word_offset<DIM>(l, h, !wideg), w);
+            if constexpr (DIM != 1) {
+                if (wideg && m.nz && lane < 2) store_u32u(wbase + m.off + m.head + 4 * lane, sm.aux[wave][2 * g + lane]);
+            }
+        });
     }
 
-    // ---- pass 2: stream words ----------------------------------------------
-    static_for<0, G>([&](auto gc) {
-        constexpr int g = decltype(gc)::value;
-        const Meta m(s_meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
-        const bool wideg = (DIM != 1) && ((big >> g) & 1);
-        uint32_t w = W[g];
-        if constexpr (MODE == kOutlier) {
-            if (m.sel && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
-        }
-        const bool st = m.nz && (l < m.R || (l == 31 && !wideg));
-        if (st) store_u32u(wbase + m.off + m.head + word_offset<DIM>(l, h, !wideg), w);
-        if constexpr (DIM != 1) {
-            if (wideg && m.nz && lane < 2) store_u32u(wbase + m.off + m.head + 4 * lane, s_sgn[wave][g][lane]);
-        }
-    });
+    tile = next;
+    if (same) continue;
+    if (tile >= ntiles) return;
+    it = src.locate(tile, it);
+    if (src.variant(it) != cur.variant) return;
+    cur = src.get(it);
+    lg.init(lane, cur.geo);
+    load_tile(tile - cur.tile0);
     }   // tiles
 }
 
 // ---------------------------------------------------------------------------
-// decompress
+// decompress: persistent loop, same contract as compress_run
 // ---------------------------------------------------------------------------
-template <int DIM, int MODE, class T, int G, int WAVES>
-__global__ void __launch_bounds__(WAVES * kWaveSize)
-decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Geom geo, const T eb,
-                  const ScanArgs sa, const uint64_t cmp_size) {
+template <int DIM, int MODE, class T, int G, int WAVES, int GMAX, class Src>
+__device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uint64_t& tile, const uint64_t ntiles,
+                                               const ScanCtx& sc, Smem<WAVES, GMAX>& sm) {
     constexpr int S = (DIM == 1) ? 4 : 8;
     constexpr int BPG = (DIM == 1) ? 2 : 1;
     constexpr int NB = BPG * G;
-    __shared__ uint64_t s_excl;
-    __shared__ uint64_t s_wtot[WAVES];
-    __shared__ uint32_t s_meta[WAVES][NB];
-    __shared__ uint32_t s_o[WAVES][NB];
+    static_assert(G <= GMAX, "shared memory sized for the largest variant");
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
-    LaneGeo<DIM> lg;
-    lg.init(lane, geo);
     XposeConsts xc;
     xc.init(lane);
+    const uint64_t stride = gridDim.x;
+    Item cur = src.get(it);
+    LaneGeo<DIM> lg;
+    lg.init(lane, cur.geo);
 
-    for (uint64_t tile = blockIdx.x; tile < geo.ntiles; tile += gridDim.x) {
-    const uint64_t grp0 = (tile * WAVES + (uint64_t)wave) * G;
+    for (;;) {
+    const Geom& geo = cur.geo;
+    const uint8_t* __restrict__ cmp = (const uint8_t*)cur.src;
+    T* __restrict__ out = (T*)cur.dst;
+    const T eb = (T)cur.param;
+    const uint64_t lt = tile - cur.tile0;
+    const uint64_t grp0 = (lt * WAVES + (uint64_t)wave) * G;
 
     // ---- bookkeeping: lane j <-> block j of this wave -----------------------
     const uint64_t blk_idx = (DIM == 1) ? 2 * grp0 + (uint64_t)lane : grp0 + (uint64_t)lane;
@@ -737,13 +993,14 @@ decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Ge
     const uint32_t incl = scan_wave(sz);
     const uint32_t off = incl - sz;
     const uint64_t wtot = rdlane(incl, kWaveSize - 1);
-    if (blk) s_meta[wave][lane] = pack_meta(off, v.R, sz != 0, v.sel, v.head);
+    if (blk) sm.meta[wave][lane] = pack_meta(off, v.R, sz != 0, v.sel, v.head);
 
-    const TileOffsets to = tile_scan<WAVES>(sa, tile, wtot, wave, lane, s_wtot, &s_excl);
-    if (tile == geo.ntiles - 1 && threadIdx.x == 0 && sa.total) *sa.total = geo.H + to.tile_incl;
+    const TileOffsets to = tile_scan<WAVES, (16 / WAVES > 1 ? 16 / WAVES : 1)>(sc, tile, cur.tile0, wtot, wave, lane, sm.wtot, sm.lb,
+        [&](uint64_t p) { return header_tile_bytes<DIM, MODE, G, WAVES>(cmp, geo, p - cur.tile0, lane); });
+    if (lt == geo.ntiles - 1 && threadIdx.x == 0 && cur.total) *cur.total = geo.H + to.tile_incl;
     const uint8_t* const wbase = cmp + geo.H + to.wave_base;
-    const bool intact = geo.H + to.wave_base + wtot <= cmp_size;
-    if (!intact && lane == 0) atomicOr(sa.err, kErrTruncated);
+    const bool intact = geo.H + to.wave_base + wtot <= cur.limit;
+    if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
     if constexpr (MODE == kOutlier) {
         // a selected block starts with nb <= 4 prefix bytes and then a 4-byte sign
         // word, so one u32 read at the block start stays inside the block
@@ -752,7 +1009,7 @@ decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Ge
             o = load_u32u(wbase + off);
             if (v.nb < 4) o &= (1u << (8 * v.nb)) - 1;
         }
-        if (lane < NB) s_o[wave][lane] = o;
+        if (lane < NB) sm.aux[wave][lane] = o;
         __builtin_amdgcn_wave_barrier();
     }
 
@@ -760,7 +1017,7 @@ decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Ge
     uint32_t W[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const Meta m(s_meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
+        const Meta m(sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
         const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.R >= 32;
         const bool ld = intact && m.nz && (l < m.R || (l == 31 && !wideg));
         W[g] = ld ? load_u32u(wbase + m.off + m.head + word_offset<DIM>(l, h, !wideg)) : 0u;
@@ -771,7 +1028,7 @@ decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Ge
     if constexpr (DIM != 1) wk.init(grp0, geo);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const Meta m(s_meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
+        const Meta m(sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
         const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.R >= 32;
         const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
         uint32_t a, neg;
@@ -784,7 +1041,7 @@ decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Ge
             a = x;
         }
         if constexpr (MODE == kOutlier) {
-            if (m.sel && l == 7 && (DIM == 1 || h == 0)) a = s_o[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
+            if (m.sel && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
         }
         const uint32_t dm = neg ? 0u - a : a;
         const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
@@ -798,7 +1055,108 @@ decompress_kernel(T* __restrict__ out, const uint8_t* __restrict__ cmp, const Ge
         if (valid) out[idx] = val;
         if constexpr (DIM != 1) wk.next(geo);
     }
+
+    tile += stride;
+    if (tile >= ntiles) return;
+    if (tile < cur.tile0 + geo.ntiles) continue;
+    it = src.locate(tile, it);
+    if (src.variant(it) != cur.variant) return;
+    cur = src.get(it);
+    lg.init(lane, cur.geo);
     }   // tiles
+}
+
+// ---------------------------------------------------------------------------
+// single-array kernels: one item, every tile of the same variant
+// ---------------------------------------------------------------------------
+template <int DIM, int MODE, class T, int G, int WAVES>
+__global__ void __launch_bounds__(WAVES * kWaveSize)
+compress_kernel(const Item item, const ScanCtx sc) {
+    __shared__ Smem<WAVES, G> sm;
+    const OneItem src{item};
+    uint32_t it = 0;
+    uint64_t tile = blockIdx.x;
+    if (tile < item.geo.ntiles) compress_run<DIM, MODE, T, G, WAVES>(src, it, tile, item.geo.ntiles, sc, sm);
+}
+
+template <int DIM, int MODE, class T, int G, int WAVES>
+__global__ void __launch_bounds__(WAVES * kWaveSize)
+decompress_kernel(const Item item, const ScanCtx sc) {
+    __shared__ Smem<WAVES, G> sm;
+    const OneItem src{item};
+    uint32_t it = 0;
+    uint64_t tile = blockIdx.x;
+    if (tile < item.geo.ntiles) decompress_run<DIM, MODE, T, G, WAVES>(src, it, tile, item.geo.ntiles, sc, sm);
+}
+
+}  // namespace cuszp
+
+namespace cuszp {
+
+// ---------------------------------------------------------------------------
+// batched kernels: the arrays of a table, of any variant, in one persistent
+// launch. Workgroup w takes tiles w, w + B, ... of the concatenated tile space;
+// a tile's array is found by binary search on tile0, and each array's tiles
+// form their own lookback segment, so every array's stream is exactly its
+// single-array stream. f64 arrays use half as many groups per wave (tiles of
+// the same byte size as f32 tiles).
+// ---------------------------------------------------------------------------
+constexpr int kBatchGF32 = 16;
+constexpr int kBatchGF64 = 8;
+
+#define CUSZP_BATCH_CASES(RUN)                                                        \
+    case 0: RUN(1, kFixed, float, kBatchGF32); break;                                 \
+    case 1: RUN(1, kPlain, float, kBatchGF32); break;                                 \
+    case 2: RUN(1, kOutlier, float, kBatchGF32); break;                               \
+    case 3: RUN(2, kFixed, float, kBatchGF32); break;                                 \
+    case 4: RUN(2, kPlain, float, kBatchGF32); break;                                 \
+    case 5: RUN(2, kOutlier, float, kBatchGF32); break;                               \
+    case 6: RUN(3, kFixed, float, kBatchGF32); break;                                 \
+    case 7: RUN(3, kPlain, float, kBatchGF32); break;                                 \
+    case 8: RUN(3, kOutlier, float, kBatchGF32); break;                               \
+    case 9: RUN(1, kFixed, double, kBatchGF64); break;                                \
+    case 10: RUN(1, kPlain, double, kBatchGF64); break;                               \
+    case 11: RUN(1, kOutlier, double, kBatchGF64); break;                             \
+    case 12: RUN(2, kFixed, double, kBatchGF64); break;                               \
+    case 13: RUN(2, kPlain, double, kBatchGF64); break;                               \
+    case 14: RUN(2, kOutlier, double, kBatchGF64); break;                             \
+    case 15: RUN(3, kFixed, double, kBatchGF64); break;                               \
+    case 16: RUN(3, kPlain, double, kBatchGF64); break;                               \
+    case 17: RUN(3, kOutlier, double, kBatchGF64); break;                             \
+    default: return;
+
+template <int WAVES>
+__global__ void __launch_bounds__(WAVES * kWaveSize)
+compress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, uint64_t ntiles, ScanCtx sc,
+                      const unsigned long long* __restrict__ epoch) {
+    __shared__ Smem<WAVES, kBatchGF32> sm;
+    sc.tag = (epoch[0] & kEpochMask) << kEpochShift;
+    const ItemTable src{items, nitems};
+    uint64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    uint32_t it = src.locate(tile, 0);
+#define CUSZP_RUN_C(D, M, T, G) compress_run<D, M, T, G, WAVES>(src, it, tile, ntiles, sc, sm)
+    while (tile < ntiles) {
+        switch (src.variant(it)) { CUSZP_BATCH_CASES(CUSZP_RUN_C) }
+    }
+#undef CUSZP_RUN_C
+}
+
+template <int WAVES>
+__global__ void __launch_bounds__(WAVES * kWaveSize)
+decompress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, uint64_t ntiles, ScanCtx sc,
+                        const unsigned long long* __restrict__ epoch) {
+    __shared__ Smem<WAVES, kBatchGF32> sm;
+    sc.tag = (epoch[0] & kEpochMask) << kEpochShift;
+    const ItemTable src{items, nitems};
+    uint64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    uint32_t it = src.locate(tile, 0);
+#define CUSZP_RUN_D(D, M, T, G) decompress_run<D, M, T, G, WAVES>(src, it, tile, ntiles, sc, sm)
+    while (tile < ntiles) {
+        switch (src.variant(it)) { CUSZP_BATCH_CASES(CUSZP_RUN_D) }
+    }
+#undef CUSZP_RUN_D
 }
 
 }  // namespace cuszp

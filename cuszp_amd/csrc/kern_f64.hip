@@ -4,48 +4,24 @@
 namespace cuszp {
 
 template <int D, int M, int W>
-static hipError_t launch_c(const double* in, uint8_t* out, const Geom& g, double r, const ScanArgs& sa,
-                           uint64_t cap, hipStream_t st) {
-    return launch_persistent(compress_kernel<D, M, double, kGroupsPerWave, W>, W * kWaveSize, g.ntiles, st,
-                             in, out, g, r, sa, cap);
+static hipError_t launch_c(const Item& item, const ScanCtx& sc, hipStream_t st) {
+    return launch_persistent(compress_kernel<D, M, double, kGroupsPerWave, W>, W * kWaveSize, item.geo.ntiles, st,
+                             item, sc);
 }
 template <int D, int M, int W>
-static hipError_t launch_d(double* out, const uint8_t* cmp, const Geom& g, double eb, const ScanArgs& sa,
-                           uint64_t cmp_size, hipStream_t st) {
-    return launch_persistent(decompress_kernel<D, M, double, kGroupsPerWave, W>, W * kWaveSize, g.ntiles, st,
-                             out, cmp, g, eb, sa, cmp_size);
+static hipError_t launch_d(const Item& item, const ScanCtx& sc, hipStream_t st) {
+    return launch_persistent(decompress_kernel<D, M, double, kGroupsPerWave, W>, W * kWaveSize, item.geo.ntiles, st,
+                             item, sc);
 }
 
-#define CUSZP_DISPATCH_W(FN, D, M, ...)                                  \
-    switch (waves) {                                                     \
-        case 4: return FN<D, M, 4>(__VA_ARGS__);                         \
-        case 8: return FN<D, M, 8>(__VA_ARGS__);                         \
-        case 16: return FN<D, M, 16>(__VA_ARGS__);                       \
-        default: return hipErrorInvalidValue;                            \
-    }
-#define CUSZP_DISPATCH(FN, ...)                                          \
-    switch (dim * 3 + mode) {                                            \
-        case 3: CUSZP_DISPATCH_W(FN, 1, kFixed, __VA_ARGS__) \
-        case 4: CUSZP_DISPATCH_W(FN, 1, kPlain, __VA_ARGS__) \
-        case 5: CUSZP_DISPATCH_W(FN, 1, kOutlier, __VA_ARGS__) \
-        case 6: CUSZP_DISPATCH_W(FN, 2, kFixed, __VA_ARGS__) \
-        case 7: CUSZP_DISPATCH_W(FN, 2, kPlain, __VA_ARGS__) \
-        case 8: CUSZP_DISPATCH_W(FN, 2, kOutlier, __VA_ARGS__) \
-        case 9: CUSZP_DISPATCH_W(FN, 3, kFixed, __VA_ARGS__) \
-        case 10: CUSZP_DISPATCH_W(FN, 3, kPlain, __VA_ARGS__) \
-        case 11: CUSZP_DISPATCH_W(FN, 3, kOutlier, __VA_ARGS__) \
-        default: return hipErrorInvalidValue;                            \
-    }
+#include "dispatch.inc"
 
-hipError_t launch_compress_f64(int dim, int mode, int waves, const double* in, uint8_t* out, const Geom& g,
-                               double eb, const ScanArgs& sa, uint64_t cap, hipStream_t st) {
-    const double r = double(0.5f) / eb;   // 0.5f/eb in double on the host (src/cuSZp_kernels_1D_f64.cu:33)
-    CUSZP_DISPATCH(launch_c, in, out, g, r, sa, cap, st)
+hipError_t launch_compress_f64(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st) {
+    CUSZP_DISPATCH(launch_c, item, sc, st)
 }
 
-hipError_t launch_decompress_f64(int dim, int mode, int waves, double* out, const uint8_t* cmp, const Geom& g,
-                                 double eb, const ScanArgs& sa, uint64_t cmp_size, hipStream_t st) {
-    CUSZP_DISPATCH(launch_d, out, cmp, g, eb, sa, cmp_size, st)
+hipError_t launch_decompress_f64(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st) {
+    CUSZP_DISPATCH(launch_d, item, sc, st)
 }
 
 }  // namespace cuszp

@@ -1,5 +1,5 @@
-// Host-side launch interface between the C ABI (cuszp_api.cpp) and the kernel
-// instantiations (kern_f32.hip, kern_f64.hip).
+// Host-side launch interface between the C ABI (cuszp_api.cpp, cuszp_batch.cpp)
+// and the kernel instantiations (kern_f32.hip, kern_f64.hip, kern_batch_*.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,9 +11,12 @@ namespace cuszp {
 
 // Groups (64-element units) per wave; a tile is waves * kGroupsPerWave groups.
 constexpr int kGroupsPerWave = 16;
+// Batched kernels: groups per wave by element type (f64 tiles hold the same
+// bytes as f32 tiles, and keep the f64 variants' registers in check).
+constexpr int kBatchGroupsF32 = 16;
+constexpr int kBatchGroupsF64 = 8;
 
-// Tile plan: waves per workgroup (4, 8 or 16) and the number of tiles.
-// Larger tiles cut the number of decoupled-lookback steps on big arrays;
+// Tile plan: waves per workgroup (4 or 8) and the number of tiles.
 // CUSZP_WAVES in the environment overrides the choice (tuning experiments).
 struct TilePlan {
     int waves;
@@ -21,19 +24,27 @@ struct TilePlan {
 };
 TilePlan plan_tiles(int dim, const Geom& g);
 
-hipError_t launch_compress_f32(int dim, int mode, int waves, const float* in, uint8_t* out, const Geom& g,
-                               float eb, const ScanArgs& sa, uint64_t cap, hipStream_t st);
-hipError_t launch_compress_f64(int dim, int mode, int waves, const double* in, uint8_t* out, const Geom& g,
-                               double eb, const ScanArgs& sa, uint64_t cap, hipStream_t st);
-hipError_t launch_decompress_f32(int dim, int mode, int waves, float* out, const uint8_t* cmp, const Geom& g,
-                                 float eb, const ScanArgs& sa, uint64_t cmp_size, hipStream_t st);
-hipError_t launch_decompress_f64(int dim, int mode, int waves, double* out, const uint8_t* cmp, const Geom& g,
-                                 double eb, const ScanArgs& sa, uint64_t cmp_size, hipStream_t st);
+// Single-array launches; item.param is 0.5/eb (compress) or eb (decompress).
+hipError_t launch_compress_f32(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st);
+hipError_t launch_compress_f64(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st);
+hipError_t launch_decompress_f32(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st);
+hipError_t launch_decompress_f64(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st);
+
+// Batched launches over a device table of items (ordered by tile0); the scan
+// tag is read from *epoch when the kernel starts (graph-replay safe).
+// `grid` = 0 picks the persistent grid from the occupancy model.
+hipError_t launch_compress_batch(int waves, unsigned grid, const Item* items, uint32_t nitems, uint64_t ntiles,
+                                 const ScanCtx& sc, const unsigned long long* epoch, hipStream_t st);
+hipError_t launch_decompress_batch(int waves, unsigned grid, const Item* items, uint32_t nitems, uint64_t ntiles,
+                                   const ScanCtx& sc, const unsigned long long* epoch, hipStream_t st);
+// Increments *epoch (one thread); runs before each batched launch.
+hipError_t launch_bump_epoch(unsigned long long* epoch, hipStream_t st);
+// Resident workgroups of a batched kernel on the whole device.
+int batch_resident(bool compress, int waves);
 
 // Workgroups of `kernel` that can be resident on one CU: the smaller of HIP's
-// occupancy query and a register model. The HIP query only sees VGPRs; these
-// kernels also use ~106 SGPRs, which caps them at 7 waves per SIMD (800 SGPRs
-// per SIMD in 16-register granules).
+// occupancy query and a register model (the HIP query only sees VGPRs; these
+// kernels also use ~106 SGPRs, which caps them at 7 waves per SIMD).
 template <class Kernel>
 int resident_per_cu(Kernel kernel, int threads) {
     int hip_est = 0;
@@ -47,14 +58,21 @@ int resident_per_cu(Kernel kernel, int threads) {
     return hip_est < own ? hip_est : own;
 }
 
-// Divisor applied to every persistent grid after a scan stall (see kErrStall);
-// the C ABI doubles it and retries, so a wrong residency estimate costs one
-// retry instead of a hang.
-int& grid_divisor(int dev);
+// Persistent grids are this many sixteenths of the occupancy estimate. The C
+// ABI lowers it by one sixteenth (down to a quarter) each time a launch reports
+// kInfoStolen, i.e. some workgroup waited on a tile whose owner was not running.
+int& grid_sixteenths(int dev);
 
-// Launch of a persistent grid: as many workgroups as can be resident at once
-// (never more than ntiles), so every tile a workgroup waits on belongs to a
-// running workgroup.
+template <class Kernel>
+int device_resident(Kernel kernel, int threads, int dev) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const int per_cu = resident_per_cu(kernel, threads);
+    return per_cu > 0 && cus > 0 ? per_cu * cus : 0;
+}
+
+// Launch of a persistent grid: at most as many workgroups as can be resident
+// at once (never more than ntiles).
 template <class Kernel, class... Args>
 hipError_t launch_persistent(Kernel kernel, int threads, uint64_t ntiles, hipStream_t st, Args... args) {
     static int max_blocks[64] = {0};   // per device
@@ -63,14 +81,10 @@ hipError_t launch_persistent(Kernel kernel, int threads, uint64_t ntiles, hipStr
     if (e != hipSuccess) return e;
     int& mb = max_blocks[dev & 63];
     if (!mb) {
-        int cus = 0;
-        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e != hipSuccess) return e;
-        const int per_cu = resident_per_cu(kernel, threads);
-        if (per_cu <= 0 || cus <= 0) return hipErrorInvalidConfiguration;
-        mb = per_cu * cus;
+        mb = device_resident(kernel, threads, dev);
+        if (mb <= 0) return hipErrorInvalidConfiguration;
     }
-    int lim = mb / grid_divisor(dev);
+    int lim = (int)((int64_t)mb * grid_sixteenths(dev) / 16);
     if (lim < 1) lim = 1;
     const unsigned grid = (unsigned)(ntiles < (uint64_t)lim ? ntiles : (uint64_t)lim);
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, st, args...);

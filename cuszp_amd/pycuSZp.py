@@ -170,3 +170,115 @@ def decompress_tensor(cmp, size: int, num_elements: int, dtype, error_bound: flo
         stream = torch.cuda.current_stream(cmp.device)
     decompress_ex(out, cmp, num_elements, size, error_bound, dim, dims, typ, m, stream)
     return out
+
+
+# ---------------------------------------------------------------------------
+# batched form: many arrays in one persistent launch (include/cuSZp.h,
+# cuSZp_batch_plan_*). Every array's stream is byte-identical to its
+# single-array stream.
+# ---------------------------------------------------------------------------
+from ._lib import BATCH_COMPRESS, BATCH_DECOMPRESS, BATCH_GRAPH, BATCH_TUNE, BatchItem  # noqa: E402
+
+
+def batch_item(d_in, d_out, num_elements: int, capacity: int, error_bound: float, dim: int = 1,
+               dims=(0, 0, 0), data_type: int = 0, mode: int = 0) -> BatchItem:
+    it = BatchItem()
+    it.d_in = _ptr(d_in).value if d_in is not None else None
+    it.d_out = _ptr(d_out).value if d_out is not None else None
+    it.nbEle = num_elements
+    it.capacity = capacity
+    it.errorBound = float(error_bound)
+    it.dim = dim
+    it.dims = Uint3(*dims)
+    it.type = data_type
+    it.mode = mode
+    return it
+
+
+class BatchPlan:
+    """Owns a cuSZp_batch_plan_t. `execute()` enqueues the whole batch (a graph
+    replay when created with graph=True); `results()` synchronises and returns
+    per-array stream lengths (compress) and statuses."""
+
+    def __init__(self, items, direction: int = BATCH_COMPRESS, graph: bool = False, tune: bool = False,
+                 stream=None):
+        self.count = len(items)
+        self._items = (BatchItem * max(self.count, 1))(*items)
+        self._plan = ctypes.c_void_p(0)
+        bad = ctypes.c_size_t(0)
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_TUNE if tune else 0)
+        rc = lib.cuSZp_batch_plan_create(ctypes.byref(self._plan), self._items, self.count, direction, flags,
+                                         _stream(stream), ctypes.byref(bad))
+        if rc:
+            raise CuSZpError(rc, f"batch plan (array {bad.value})")
+        self.direction = direction
+
+    def execute(self, stream=None) -> None:
+        rc = lib.cuSZp_batch_plan_execute(self._plan, _stream(stream))
+        if rc:
+            raise CuSZpError(rc, "batch execute")
+
+    def results(self, stream=None, check: bool = True):
+        sizes = (ctypes.c_size_t * max(self.count, 1))()
+        st = (ctypes.c_int * max(self.count, 1))()
+        rc = lib.cuSZp_batch_plan_results(self._plan, sizes, st, _stream(stream))
+        if rc and check:
+            raise CuSZpError(rc, "batch results")
+        return list(sizes)[: self.count], list(st)[: self.count]
+
+    def tune(self, stream=None):
+        w = ctypes.c_int(0)
+        g = ctypes.c_uint(0)
+        rc = lib.cuSZp_batch_plan_tune(self._plan, _stream(stream), ctypes.byref(w), ctypes.byref(g))
+        if rc:
+            raise CuSZpError(rc, "batch tune")
+        return w.value, g.value
+
+    def device_sizes(self) -> int:
+        return lib.cuSZp_batch_plan_device_sizes(self._plan) or 0
+
+    def close(self) -> None:
+        if self._plan:
+            lib.cuSZp_batch_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p(0)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def compress_tensors(tensors, error_bounds, modes, dims_list=None, dims_kind=None, stream=None, graph=False,
+                     tune=False):
+    """Compress a list of contiguous float32/float64 CUDA tensors in one batched
+    launch. error_bounds / modes: one per tensor (absolute bounds; mode names or
+    ints). dims_kind[i] in {1, 2, 3} with dims_list[i] = (x, y, z) for 2D/3D.
+    Returns [(uint8 tensor, size), ...]."""
+    import torch
+
+    n = len(tensors)
+    dims_kind = dims_kind or [1] * n
+    dims_list = dims_list or [(0, 0, 0)] * n
+    outs, items = [], []
+    for t, eb, m, dk, dd in zip(tensors, error_bounds, modes, dims_kind, dims_list):
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        mi = MODES[m] if isinstance(m, str) else int(m)
+        typ = _type_of(t)
+        cap = max_compressed_size(t.numel(), dk, tuple(dd), typ, mi)
+        out = torch.empty(max(cap, 1), dtype=torch.uint8, device=t.device)
+        outs.append(out)
+        items.append(batch_item(t, out, t.numel(), cap, eb, dk, tuple(dd), typ, mi))
+    if stream is None and n:
+        stream = torch.cuda.current_stream(tensors[0].device)
+    with BatchPlan(items, BATCH_COMPRESS, graph=graph, tune=tune, stream=stream) as plan:
+        plan.execute(stream)
+        sizes, _ = plan.results(stream)
+    return list(zip(outs, sizes))

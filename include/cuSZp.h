@@ -159,6 +159,55 @@ int cuSZp_last_async_status(hipStream_t stream);
 
 const char* cuSZp_status_string(int status);
 
+/* ---- batched form (no reference counterpart) ----
+ * Many arrays of any sizes, dtypes, modes, dims and error bounds compress (or
+ * decompress) in ONE persistent kernel launch. Every array's stream is
+ * byte-identical to its single-array stream. A plan validates the arrays once,
+ * uploads their descriptors, owns the scan state and can hold the launch as a
+ * captured hipGraph, so execute() is a single graph launch. */
+typedef struct {
+    const void* d_in;       /* compress: values; decompress: compressed stream */
+    void* d_out;            /* compress: stream buffer; decompress: values */
+    size_t nbEle;
+    size_t capacity;        /* compress: bytes at d_out; decompress: stream length (cmpSize) */
+    double errorBound;      /* absolute; f32 arrays use (float)errorBound, like cuSZp_compress_ex */
+    cuszp_dim_t dim;
+    cuszp_dims_t dims;      /* ignored for 1D */
+    cuszp_type_t type;
+    cuszp_mode_t mode;
+} cuszp_batch_item_t;
+
+typedef struct cuszp_batch_plan_s* cuszp_batch_plan_t;
+
+enum { CUSZP_BATCH_COMPRESS = 0, CUSZP_BATCH_DECOMPRESS = 1 };
+enum {
+    CUSZP_BATCH_GRAPH = 1,  /* capture the launch in a hipGraph; execute() replays it */
+    CUSZP_BATCH_TUNE = 2    /* autotune the launch shape when the plan is created (runs the batch) */
+};
+
+/* Validates every item (status of the first bad one is returned, its index in
+ * *bad_index if non-null), allocates the plan's device state on the current
+ * device and enqueues its set-up on `stream`. */
+int cuSZp_batch_plan_create(cuszp_batch_plan_t* plan, const cuszp_batch_item_t* items, size_t count,
+                            int direction, int flags, hipStream_t stream, size_t* bad_index);
+/* Enqueues the batch on `stream` and returns (no host synchronisation). */
+int cuSZp_batch_plan_execute(cuszp_batch_plan_t plan, hipStream_t stream);
+/* Synchronises `stream`; copies each array's stream length (compress) and
+ * status to the host arrays (either may be null); returns the first non-zero
+ * status or CUSZP_SUCCESS. */
+int cuSZp_batch_plan_results(cuszp_batch_plan_t plan, size_t* cmpSizes, int* statuses, hipStream_t stream);
+/* Device array of per-array stream lengths written by each execute (compress). */
+const unsigned long long* cuSZp_batch_plan_device_sizes(cuszp_batch_plan_t plan);
+/* Times the candidate launch shapes on `stream` (runs the batch several times)
+ * and keeps the fastest; returns its workgroup size in *waves and grid in *grid. */
+int cuSZp_batch_plan_tune(cuszp_batch_plan_t plan, hipStream_t stream, int* waves, unsigned* grid);
+void cuSZp_batch_plan_destroy(cuszp_batch_plan_t plan);
+
+/* One-shot batched calls: create a plan, execute, read results, destroy. */
+int cuSZp_compress_batch(const cuszp_batch_item_t* items, size_t count, size_t* cmpSizes, int* statuses,
+                         hipStream_t stream);
+int cuSZp_decompress_batch(const cuszp_batch_item_t* items, size_t count, int* statuses, hipStream_t stream);
+
 /* Library version string ("cuszp_amd <semver> gfx950"). */
 const char* cuSZp_version(void);
 

@@ -102,3 +102,34 @@ def test_status_strings(L):
     for s in range(8):
         assert L.cuSZp_status_string(s)
     assert b"gfx950" in L.cuSZp_version()
+
+
+def test_batch_item_layout():
+    # cuszp_batch_item_t: 5 eight-byte fields, dim, dims{x,y,z}, type, mode
+    B = _lib.BatchItem
+    assert ctypes.sizeof(B) == 64
+    assert [B.dim.offset, B.dims.offset, B.type.offset, B.mode.offset] == [40, 44, 56, 60]
+    src = open(os.path.join(ROOT, "include", "cuSZp.h")).read()
+    for name, val in [("CUSZP_BATCH_COMPRESS", 0), ("CUSZP_BATCH_DECOMPRESS", 1), ("CUSZP_BATCH_GRAPH", 1),
+                      ("CUSZP_BATCH_TUNE", 2)]:
+        assert re.search(rf"{name}\s*=\s*{val}\b", src), name
+
+
+def test_batch_plan_rejects_before_device_work(L):
+    # validation runs before any allocation or launch: no GPU needed
+    U = _lib.Uint3
+    good = _lib.BatchItem(16, 16, 1000, 1 << 20, 1e-3, 1, U(0, 0, 0), 0, 1)
+    bad_eb = _lib.BatchItem(16, 16, 1000, 1 << 20, -1.0, 1, U(0, 0, 0), 0, 1)
+    bad_dims = _lib.BatchItem(16, 16, 10, 1 << 20, 1e-3, 2, U(5, 2, 1), 0, 1)
+    small = _lib.BatchItem(16, 16, 100_000, 10, 1e-3, 1, U(0, 0, 0), 0, 1)
+    for items, status, index in [([good, bad_eb], 1, 1), ([bad_dims], 2, 0), ([good, good, small], 4, 2)]:
+        arr = (_lib.BatchItem * len(items))(*items)
+        plan = ctypes.c_void_p(0)
+        bad = ctypes.c_size_t(99)
+        rc = L.cuSZp_batch_plan_create(ctypes.byref(plan), arr, len(items), 0, 0, None, ctypes.byref(bad))
+        assert rc == status and bad.value == index and not plan.value
+    # decompress: a stream shorter than its header is corrupt
+    arr = (_lib.BatchItem * 1)(small)
+    plan = ctypes.c_void_p(0)
+    assert L.cuSZp_batch_plan_create(ctypes.byref(plan), arr, 1, 1, 0, None, None) == 5
+    assert L.cuSZp_batch_plan_create(ctypes.byref(plan), arr, 1, 7, 0, None, None) == 1   # bad direction
