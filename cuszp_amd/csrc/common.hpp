@@ -56,6 +56,8 @@ inline int make_geom(int dim, size_t n, cuszp_dims_t d, Geom& g) {
         g.Bx = (d.x + 3) / 4; g.By = (d.y + 3) / 4; bz = (d.z + 3) / 4;
     }
     g.nblocks = (uint64_t)g.Bx * g.By * bz;
+    g.aligned = (dim == CUSZP_DIM_2D) ? (d.x % 8 == 0 && d.y % 8 == 0)
+                                      : (d.x % 4 == 0 && d.y % 4 == 0 && d.z % 4 == 0);
     if (g.nblocks >= (1ull << 40)) return CUSZP_ERROR_TOO_LARGE;
     g.H = (g.nblocks + 3) / 4 * 4;                  // src/cuSZp_kernels_2D_f32.cu:46
     return CUSZP_SUCCESS;

@@ -80,6 +80,7 @@ struct Geom {
     uint64_t XY;        // X*Y
     uint32_t X, Y, Z;
     uint32_t Bx, By;    // blocks along x and y (2D/3D)
+    uint32_t aligned;   // 2D/3D: every block lies entirely inside the array (extents multiples of the block)
 };
 
 // Launch-wide scan state.
@@ -136,6 +137,12 @@ __device__ __forceinline__ uint32_t wrlane(uint32_t v, uint32_t old) {
     return old;
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+// A value the compiler cannot prove wave-uniform but that is (kept in SGPRs,
+// so that uniform-base + 32-bit-offset addressing applies).
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    return (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v) |
+           ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32);
+}
 
 // ---------------------------------------------------------------------------
 // 32x32 bit transpose within each 32-lane half: out lane j bit i = in lane i bit j.
@@ -572,6 +579,13 @@ __device__ __forceinline__ bool elem_of(const Geom& g, const LaneGeo<DIM>& lg, c
     }
 }
 
+// Element index of the first element of the walker's block (2D/3D).
+template <int DIM>
+__device__ __forceinline__ uint64_t block_origin(const Geom& g, const Walker& wk) {
+    if constexpr (DIM == 2) return ((uint64_t)wk.bz * g.Y + (uint64_t)wk.by * 8) * g.X + (uint64_t)wk.bx * 8;
+    else return ((uint64_t)wk.bz * 4 * g.Y + (uint64_t)wk.by * 4) * g.X + (uint64_t)wk.bx * 4;
+}
+
 // Predecessor lane of element e in the block's Lorenzo chain (-1: none).
 template <int DIM>
 __device__ __forceinline__ int pred_lane(int e) {
@@ -613,18 +627,29 @@ __device__ __forceinline__ uint32_t inverse_lorenzo(uint32_t d, int e) {
 }
 
 // ---------------------------------------------------------------------------
-// per-block metadata handed from the bookkeeping lanes to pass 2 through LDS:
-//   [19:0] payload offset from the wave's first block, [25:20] planes (<= 32),
-//   [26] block has a payload, [27] outlier-selected, [30:28] prefix bytes
+// per-block metadata handed from the bookkeeping lanes to the stream-word
+// lanes through LDS (one 8-byte entry per block):
+//   a: [19:0] byte offset of the block's sign word from the wave's first
+//      block (past any outlier prefix), [30] 32 planes (2D/3D: the sign words
+//      move in front of the planes), [31] outlier-selected
+//   b: bit l set when lane l of a 32-lane half carries one of the block's
+//      stream words (planes 0..R-1 in lanes 0..R-1, the sign word in lane 31)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t pack_meta(uint32_t off, uint32_t R, bool nz, uint32_t sel, uint32_t head) {
-    return off | ((R < 32 ? R : 32u) << 20) | ((uint32_t)nz << 26) | (sel << 27) | (head << 28);
-}
 struct Meta {
-    uint32_t off, R, nz, sel, head;
-    __device__ __forceinline__ explicit Meta(uint32_t m)
-        : off(m & 0xfffff), R((m >> 20) & 63), nz((m >> 26) & 1), sel((m >> 27) & 1), head((m >> 28) & 7) {}
+    uint32_t a, b;
+    __device__ __forceinline__ uint32_t off() const { return a & 0xfffffu; }
+    __device__ __forceinline__ bool wide() const { return (a >> 30) & 1; }
+    __device__ __forceinline__ bool sel() const { return (int32_t)a < 0; }
+    __device__ __forceinline__ bool nz() const { return b != 0; }
+    __device__ __forceinline__ bool has(uint32_t l) const { return (b >> l) & 1; }
 };
+__device__ __forceinline__ Meta make_meta(uint32_t off, const BlockView& v, bool nz) {
+    Meta m;
+    const bool wide = v.R >= 32;
+    m.a = (off + v.head) | ((uint32_t)wide << 30) | (v.sel << 31);
+    m.b = !nz ? 0u : wide ? ~0u : (((1u << v.R) - 1u) | 0x80000000u);
+    return m;
+}
 
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -653,7 +678,7 @@ template <int WAVES, int GMAX>
 struct Smem {
     LookbackShared<WAVES> lb;
     uint64_t wtot[WAVES];
-    uint32_t meta[WAVES][2 * GMAX];   // per-block metadata (pack_meta), one entry per block of a wave
+    Meta meta[WAVES][2 * GMAX];       // per-block metadata, one entry per block of a wave
     uint32_t aux[WAVES][2 * GMAX];    // compress: sign words of wide 2D/3D groups; decompress: outlier |d0|
 };
 
@@ -790,6 +815,8 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     const int pl_src = pl < 0 ? lane : pl;
     XposeConsts xc;
     xc.init(lane);
+    const uint32_t wo = word_offset<DIM>(l, h, true);        // this lane's word in a block's payload
+    const uint32_t wo_wide = word_offset<DIM>(l, h, false);  // same, 32-plane 2D/3D layout
     // after the transpose, element 0 of a block is bit 7 of its plane words: in
     // both halves for 1D (two blocks), only in the low half for 2D/3D
     const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
@@ -799,8 +826,10 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     LaneGeo<DIM> lg;
     lg.init(lane, cur.geo);
 
-    // All loads of a tile's G groups per wave are issued back to back; the next
-    // tile's loads (same item) are issued before the current tile's scan and stores.
+    // Loads are software-pipelined across tiles: as soon as pass 1 has consumed
+    // group g of the current tile, the same register receives group g of the
+    // next tile of this item, so every wave keeps its loads in flight through
+    // pass 1, the scan and the stores.
     T xv[G];
     uint64_t vmask = 0;   // bit g: this lane's element of group g lies inside the array
     auto load_tile = [&](uint64_t lt) {
@@ -830,13 +859,42 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     uint32_t W[G];
     uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
     uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
-    // Pass 1 has two instantiations: tiles entirely inside the array skip the
-    // per-element validity select (fixed mode never needs it: x = 0 gives q = 0).
+    const uint64_t next = tile + stride;
+    const bool same = next < cur.tile0 + geo.ntiles;   // next tile in this item
+    const T* __restrict__ in = (const T*)cur.src;
+    const uint64_t gn0 = ((next - cur.tile0) * WAVES + (uint64_t)wave) * G;
+    // The next tile's groups load during pass 1 when all of this wave's groups
+    // of it lie inside the array (then no per-element checks are needed);
+    // otherwise the tile loads after pass 1 with checks.
+    bool roll = same;
+    uint64_t nbase = 0;   // element index of the wave's first element of the next tile (2D/3D: block origin)
+    if constexpr (DIM == 1) {
+        roll = roll && (gn0 + G) * 64 <= geo.n;
+        nbase = gn0 * 64;
+    } else {
+        roll = roll && geo.aligned && gn0 + G <= geo.nblocks;
+    }
+    Walker wkn;
+    if constexpr (DIM != 1) {
+        if (roll) wkn.init(gn0, geo);
+    }
+    // Tiles entirely inside the array skip the per-element validity select
+    // (fixed mode never needs it: x = 0 gives q = 0). 1D instantiates pass 1
+    // twice for that; 2D/3D branch per group (two copies cost more registers).
+    const bool full = ballot(vmask != (G == 64 ? ~0ull : (1ull << G) - 1)) == 0;
     auto pass1 = [&](auto fullc) {
     constexpr bool FULL = decltype(fullc)::value;
     static_for<0, G>([&](auto gc) {
         constexpr int g = decltype(gc)::value;
         const uint32_t q = quantize<T>(xv[g], r);   // x = 0 outside the array gives q = 0
+        if (roll) {
+            if constexpr (DIM == 1) {
+                xv[g] = __builtin_nontemporal_load(in + nbase + 64 * g + lane);
+            } else {
+                xv[g] = __builtin_nontemporal_load(in + block_origin<DIM>(geo, wkn) + lg.off);
+                wkn.next(geo);
+            }
+        }
         uint32_t d;
         if constexpr (MODE == kFixed) {
             d = q;
@@ -849,7 +907,9 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
             qp = (pl >= 0) ? qp : 0u;
             d = q - qp;
         }
-        if constexpr (!FULL && MODE != kFixed) d = ((vmask >> g) & 1) ? d : 0u;
+        if constexpr (MODE != kFixed && !FULL) {
+            if (DIM == 1 || !full) d = ((vmask >> g) & 1) ? d : 0u;
+        }
         const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
         const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
         uint64_t m1, m2 = 0;
@@ -892,12 +952,10 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
         }
     });
     };
-    if (ballot(vmask != (G == 64 ? ~0ull : (1ull << G) - 1)) == 0) pass1(std::true_type{});
+    if (DIM == 1 && full) pass1(std::true_type{});
     else pass1(std::false_type{});
-
-    const uint64_t next = tile + stride;
-    const bool same = next < cur.tile0 + geo.ntiles;   // next tile in this item: prefetch it now
-    if (same) load_tile(next - cur.tile0);             // in flight during scan + stores
+    if (roll) vmask = (G == 64) ? ~0ull : (1ull << G) - 1;
+    else if (same) load_tile(next - cur.tile0);   // in flight during scan + stores
 
     // ---- bookkeeping: lane j <-> block j of this wave -----------------------
     uint8_t* __restrict__ out = (uint8_t*)cur.dst;
@@ -912,7 +970,7 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     const uint32_t off = incl - sz;
     const uint64_t wtot = rdlane(incl, kWaveSize - 1);
     if (blk) {
-        sm.meta[wave][lane] = pack_meta(off, v.R, sz != 0, v.sel, v.head);
+        sm.meta[wave][lane] = make_meta(off, v, sz != 0);
         // header byte; 2D/3D pad bytes [nblocks, H) are written as 0
         if (DIM == 1 || blk_idx < geo.H) out[blk_idx] = (uint8_t)byte;
     }
@@ -920,7 +978,7 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     const TileOffsets to = tile_scan<WAVES, (8 / WAVES > 1 ? 8 / WAVES : 1)>(sc, tile, cur.tile0, wtot, wave, lane, sm.wtot, sm.lb,
         [&](uint64_t p) { return compress_tile_bytes<DIM, MODE, T, G, WAVES>((const T*)cur.src, geo, r, p - cur.tile0, lane); });
     if (lt == geo.ntiles - 1 && threadIdx.x == 0 && cur.total) *cur.total = geo.H + to.tile_incl;
-    uint8_t* const wbase = out + geo.H + to.wave_base;
+    uint8_t* const wbase = out + uniform64(geo.H + to.wave_base);
     const bool fits = geo.H + to.wave_base + wtot <= cur.limit;   // never write past the caller's buffer
     if (!fits && lane == 0) atomicOr(cur.err, kErrOverflow);
     if (fits) {
@@ -930,16 +988,15 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
         // ---- pass 2: stream words --------------------------------------------
         static_for<0, G>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
-            const Meta m(sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
+            const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
             const bool wideg = (DIM != 1) && ((big >> g) & 1);
             uint32_t w = W[g];
             if constexpr (MODE == kOutlier) {
-                if (m.sel && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
+                if (m.sel() && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
             }
-            const bool st = m.nz && (l < m.R || (l == 31 && !wideg));
-            if (st) store_u32u(wbase + m.off + m.head + word_offset<DIM>(l, h, !wideg), w);
+            if (m.has(l)) store_u32u(wbase + (m.off() + (wideg ? wo_wide : wo)), w);
             if constexpr (DIM != 1) {
-                if (wideg && m.nz && lane < 2) store_u32u(wbase + m.off + m.head + 4 * lane, sm.aux[wave][2 * g + lane]);
+                if (wideg && m.nz() && lane < 2) store_u32u(wbase + (m.off() + 4 * lane), sm.aux[wave][2 * g + lane]);
             }
         });
     }
@@ -971,10 +1028,20 @@ __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uin
     const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
     XposeConsts xc;
     xc.init(lane);
+    const uint32_t wo = word_offset<DIM>(l, h, true);        // this lane's word in a block's payload
+    const uint32_t wo_wide = word_offset<DIM>(l, h, false);  // same, 32-plane 2D/3D layout
     const uint64_t stride = gridDim.x;
     Item cur = src.get(it);
     LaneGeo<DIM> lg;
     lg.init(lane, cur.geo);
+    // header byte of this lane's block in a tile (lane j <-> block j of the wave)
+    auto header_byte = [&](uint64_t lt) -> uint32_t {
+        const uint64_t g0 = (lt * WAVES + (uint64_t)wave) * G;
+        const uint64_t b = (DIM == 1) ? 2 * g0 + (uint64_t)lane : g0 + (uint64_t)lane;
+        const bool ok = (lane < NB) && (DIM == 1 || b < cur.geo.nblocks);
+        return ok ? (uint32_t)((const uint8_t*)cur.src)[b] : 0u;
+    };
+    uint32_t byte = header_byte(tile - cur.tile0);
 
     for (;;) {
     const Geom& geo = cur.geo;
@@ -987,18 +1054,19 @@ __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uin
     // ---- bookkeeping: lane j <-> block j of this wave -----------------------
     const uint64_t blk_idx = (DIM == 1) ? 2 * grp0 + (uint64_t)lane : grp0 + (uint64_t)lane;
     const bool blk = (lane < NB) && (DIM == 1 || blk_idx < geo.nblocks);
-    const uint32_t byte = blk ? (uint32_t)cmp[blk_idx] : 0u;
     const BlockView v = view_of<MODE, S>(byte);
     const uint32_t sz = blk ? v.size : 0u;
     const uint32_t incl = scan_wave(sz);
     const uint32_t off = incl - sz;
     const uint64_t wtot = rdlane(incl, kWaveSize - 1);
-    if (blk) sm.meta[wave][lane] = pack_meta(off, v.R, sz != 0, v.sel, v.head);
+    // every entry is rewritten: blocks past the end get an empty lane mask, so no
+    // load below ever uses an offset left over from an earlier tile or array
+    if (lane < NB) sm.meta[wave][lane] = make_meta(off, v, blk && sz != 0);
 
     const TileOffsets to = tile_scan<WAVES, (16 / WAVES > 1 ? 16 / WAVES : 1)>(sc, tile, cur.tile0, wtot, wave, lane, sm.wtot, sm.lb,
         [&](uint64_t p) { return header_tile_bytes<DIM, MODE, G, WAVES>(cmp, geo, p - cur.tile0, lane); });
     if (lt == geo.ntiles - 1 && threadIdx.x == 0 && cur.total) *cur.total = geo.H + to.tile_incl;
-    const uint8_t* const wbase = cmp + geo.H + to.wave_base;
+    const uint8_t* const wbase = cmp + uniform64(geo.H + to.wave_base);
     const bool intact = geo.H + to.wave_base + wtot <= cur.limit;
     if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
     if constexpr (MODE == kOutlier) {
@@ -1013,14 +1081,17 @@ __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uin
         __builtin_amdgcn_wave_barrier();
     }
 
+    // the next tile's header bytes load while this tile decodes
+    const uint64_t next = tile + stride;
+    const bool same = next < cur.tile0 + geo.ntiles;
+
     // ---- prefetch the stream words of all G groups ---------------------------
     uint32_t W[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const Meta m(sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
-        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.R >= 32;
-        const bool ld = intact && m.nz && (l < m.R || (l == 31 && !wideg));
-        W[g] = ld ? load_u32u(wbase + m.off + m.head + word_offset<DIM>(l, h, !wideg)) : 0u;
+        const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
+        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
+        W[g] = (intact && m.has(l)) ? load_u32u(wbase + (m.off() + (wideg ? wo_wide : wo))) : 0u;
     }
 
     // ---- decode ---------------------------------------------------------------
@@ -1028,27 +1099,27 @@ __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uin
     if constexpr (DIM != 1) wk.init(grp0, geo);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        const Meta m(sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)]);
-        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.R >= 32;
+        const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
+        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
         const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
         uint32_t a, neg;
         if (!wideg) {
             neg = x >> 31;
             a = x & 0x7fffffffu;
         } else {   // 32 planes: the sign words sit in their own two words
-            const uint32_t sw = (intact && m.nz) ? load_u32u(wbase + m.off + 4 * h) : 0u;
+            const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
             neg = (sw >> l) & 1;
             a = x;
         }
         if constexpr (MODE == kOutlier) {
-            if (m.sel && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
+            if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
         }
         const uint32_t dm = neg ? 0u - a : a;
         const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
         uint32_t q;
         if constexpr (MODE == kFixed) q = d;
         else q = inverse_lorenzo<DIM>(d, lane);
-        const bool zero_blk = !intact || !m.nz;
+        const bool zero_blk = !intact || !m.nz();
         const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
         uint64_t idx;
         const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
@@ -1056,13 +1127,15 @@ __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uin
         if constexpr (DIM != 1) wk.next(geo);
     }
 
-    tile += stride;
+    tile = next;
     if (tile >= ntiles) return;
-    if (tile < cur.tile0 + geo.ntiles) continue;
+    if (!same) {
     it = src.locate(tile, it);
     if (src.variant(it) != cur.variant) return;
     cur = src.get(it);
     lg.init(lane, cur.geo);
+    }
+    byte = header_byte(tile - cur.tile0);
     }   // tiles
 }
 
