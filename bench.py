@@ -134,6 +134,9 @@ def main():
     ap.add_argument("--workload", default="3d_outlier_f32", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -144,6 +147,10 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        copy_calibration(torch, args.steps)
+        return
 
     from cuszp_amd import _lib
 
@@ -281,6 +288,22 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(torch, steps):
+    """A plain copy of 2^28 f32 (1 GiB read + 1 GiB written per launch): profiling
+    its FETCH_SIZE / WRITE_SIZE gives the factor between those counters and bytes."""
+    dev = torch.device("cuda:0")
+    a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4,
+                      "copy_GBps": round(2 * a.numel() * 4 / dt / 1e9, 2)}), flush=True)
 
 
 def cpu_baseline(np, x, wl, eb, s, budget_s):

@@ -132,7 +132,7 @@ TilePlan plan_tiles(int dim, const Geom& g) {
     const uint64_t groups = (dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;   // 64-element units
     int waves = forced;
     if (!waves) waves = 4;
-    const uint64_t per_tile = (uint64_t)waves * kGroupsPerWave;
+    const uint64_t per_tile = (uint64_t)waves * groups_per_wave(waves);
     return TilePlan{waves, (groups + per_tile - 1) / per_tile};
 }
 

@@ -333,6 +333,8 @@ struct LookbackShared {
 
 // Exclusive prefix of `tile` within its segment (tiles first .. tile-1), run by
 // every wave of the workgroup (uniform control flow across the workgroup).
+// `published`: the caller already published the tile's aggregate (or, for the
+// segment's first tile, its inclusive prefix) with publish_tile().
 //
 // The workgroup publishes the tile aggregate, then reads predecessors a window
 // of WAVES * 64 * K at a time: wave v, lane j, word k reads position
@@ -352,22 +354,24 @@ struct LookbackShared {
 // later store of the same aggregate (or of its prefix) is harmless.
 template <int WAVES, int K, class AggF>
 __device__ __forceinline__ uint64_t wg_lookback(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t agg,
-                                                int wave, int lane, LookbackShared<WAVES>& ls, AggF&& agg_of) {
+                                                int wave, int lane, LookbackShared<WAVES>& ls, AggF&& agg_of,
+                                                bool published = false, const uint64_t (*pre)[K] = nullptr) {
     constexpr int SPAN = 64 * K;           // positions per wave and window
     constexpr int WIN = WAVES * SPAN;      // positions per window
     const uint64_t tag = sc.tag;
     const bool leader = (wave == 0) && (lane == 0);
     if (tile == first) {
-        if (leader) st_status(sc.status + tile, tag | kFlagPrefix | agg);
+        if (leader && !published) st_status(sc.status + tile, tag | kFlagPrefix | agg);
         return 0;
     }
-    if (leader) st_status(sc.status + tile, tag | kFlagAgg | agg);
+    if (leader && !published) st_status(sc.status + tile, tag | kFlagAgg | agg);
     const int pbase = wave * SPAN;
     uint64_t excl = 0;
     int64_t hi = (int64_t)tile;
     uint32_t spins = 0, round = 0;
     const uint64_t t_wait = __builtin_amdgcn_s_memrealtime();
     bool steal = false, stalled = false;
+    bool use_pre = pre != nullptr;
     for (;;) {
         uint64_t w[K];
         bool ready[K];
@@ -377,10 +381,11 @@ __device__ __forceinline__ uint64_t wg_lookback(const ScanCtx& sc, uint64_t tile
             w[k] = tag | kFlagPrefix;   // before the segment: inclusive prefix 0
             ready[k] = true;
             if (idx >= (int64_t)first) {
-                w[k] = ld_status(sc.status + idx);
+                w[k] = use_pre ? (*pre)[k] : ld_status(sc.status + idx);
                 ready[k] = status_ready(w[k], tag);
             }
         }
+        use_pre = false;   // only the first window comes pre-loaded
         int P;
         for (;;) {
             int Pv = WIN, Nv = WIN;
@@ -482,6 +487,63 @@ __device__ __forceinline__ uint64_t wg_lookback(const ScanCtx& sc, uint64_t tile
     }
     if (leader) st_status(sc.status + tile, tag | kFlagPrefix | (excl + agg));
     return excl;
+}
+
+// Early publication of a tile's aggregate (the segment's first tile publishes
+// its inclusive prefix): done as soon as its byte count is known.
+__device__ __forceinline__ void publish_tile(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t agg) {
+    st_status(sc.status + tile, sc.tag | (tile == first ? kFlagPrefix : kFlagAgg) | agg);
+}
+
+// Loads of the first lookback window of `tile` (the words wg_lookback reads
+// at positions p = 64*(K*wave + k) + lane), issued ahead of time: in a
+// pipelined loop they are issued before the next tile's data loads, so waiting
+// for them does not wait for that data. Positions before the segment load the
+// segment's first word instead (wg_lookback replaces them).
+template <int WAVES, int K>
+__device__ __forceinline__ void lookback_issue(const ScanCtx& sc, uint64_t tile, uint64_t first, int wave, int lane,
+                                               uint64_t (&w)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        int64_t idx = (int64_t)tile - 1 - (wave * 64 * K + 64 * k + lane);
+        if (idx < (int64_t)first) idx = (int64_t)first;
+        w[k] = ld_status(sc.status + idx);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stores of the hot loops, written as inline asm: the compiler's s_waitcnt
+// insertion does not see them, so its waits for the loop's own loads never
+// wait for these stores to complete (gfx9 counts stores in vmcnt and retires
+// memory operations in order). The loops never read back what they store.
+// Base addresses are wave-uniform (SGPRs), offsets 32-bit per lane.
+//
+// Each starts with s_nop 4: a VMEM instruction that reads an SGPR written by a
+// VALU instruction (here the v_readfirstlane that makes the base uniform) needs
+// 5 wait states on gfx9, and the compiler's hazard recognizer cannot see that
+// an inline asm block is a VMEM instruction.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void st_async_b32(const void* base, uint32_t off, uint32_t v) {
+    asm volatile("s_nop 4\n\tglobal_store_dword %0, %1, %2" : : "v"(off), "v"(v), "s"(uniform64((uint64_t)base)) : "memory");
+}
+__device__ __forceinline__ void st_async_b16(const void* base, uint32_t off, uint32_t v) {
+    asm volatile("s_nop 4\n\tglobal_store_short %0, %1, %2" : : "v"(off), "v"(v), "s"(uniform64((uint64_t)base)) : "memory");
+}
+__device__ __forceinline__ void st_async_b8(const void* base, uint32_t off, uint32_t v) {
+    asm volatile("s_nop 4\n\tglobal_store_byte %0, %1, %2" : : "v"(off), "v"(v), "s"(uniform64((uint64_t)base)) : "memory");
+}
+// little-endian 1..4 bytes
+__device__ __forceinline__ void st_async_le(const void* base, uint32_t off, uint32_t v, uint32_t nb) {
+    if (nb == 4) {
+        st_async_b32(base, off, v);
+    } else {
+        if (nb & 2) { st_async_b16(base, off, v); off += 2; v >>= 16; }
+        if (nb & 1) st_async_b8(base, off, v);
+    }
+}
+// status word publication (relaxed, agent scope: sc1 like __hip_atomic_store)
+__device__ __forceinline__ void st_status_async(const unsigned long long* status, uint32_t tile, uint64_t v) {
+    asm volatile("s_nop 4\n\tglobal_store_dwordx2 %0, %1, %2 sc1" : : "v"(tile * 8u), "v"(v), "s"(uniform64((uint64_t)status)) : "memory");
 }
 
 // Workgroup-level: combine the wave totals and run the lookback; returns
@@ -671,17 +733,6 @@ __device__ __forceinline__ uint32_t word_offset(uint32_t l, uint32_t h, bool sig
     }
 }
 
-// ---------------------------------------------------------------------------
-// Shared memory of one workgroup, sized for the largest variant of a launch.
-// ---------------------------------------------------------------------------
-template <int WAVES, int GMAX>
-struct Smem {
-    LookbackShared<WAVES> lb;
-    uint64_t wtot[WAVES];
-    Meta meta[WAVES][2 * GMAX];       // per-block metadata, one entry per block of a wave
-    uint32_t aux[WAVES][2 * GMAX];    // compress: sign words of wide 2D/3D groups; decompress: outlier |d0|
-};
-
 // Item sources: where a persistent loop finds the array a tile belongs to.
 // OneItem serves the single-array kernels (the descriptor is a kernel
 // argument); ItemTable serves the batched kernels (descriptors in device
@@ -796,15 +847,46 @@ __device__ uint64_t header_tile_bytes(const uint8_t* __restrict__ cmp, const Geo
 // ---------------------------------------------------------------------------
 // compress: persistent loop over the tiles of consecutive items of one variant
 // ---------------------------------------------------------------------------
+// Shared memory of a compress workgroup, sized for the largest variant. Three
+// tiles are in flight per workgroup (see compress_run); everything a tile needs
+// between its pass 1 and its stores lives here, indexed by tile slot.
+template <int WAVES, int GMAX>
+struct SmemC {
+    LookbackShared<WAVES> lb;
+    uint64_t wtot[3][WAVES];                    // per-wave byte counts
+    Meta meta[3][WAVES][2 * GMAX];              // per-block metadata
+    uint32_t aux[3][WAVES][2 * GMAX];           // sign words of 32-plane 2D/3D groups
+    uint32_t words[3][WAVES][GMAX][kWaveSize];  // stream words
+    uint32_t lane[3][WAVES][2 * GMAX][2];       // per block: {off | prefix bytes << 20 | header byte << 24, |d0|}
+    uint32_t wave[3][WAVES][4];                 // {wave exclusive offset in the tile, wave bytes, 32-plane groups, -}
+};
+
 // Processes tiles tile, tile + gridDim.x, ... while they belong to items of this
 // variant; returns with `tile` at the first tile it did not process and `it` at
 // that tile's item (or tile >= ntiles).
+//
+// Two-deep software pipeline built around gfx9's in-order vmcnt (stores and
+// loads share one counter, so a wait for any load also waits for every older
+// store): iteration i
+//   1. resolves the lookback of t_{i-2} (its first window was loaded at the
+//      start of iteration i-1) and writes t_{i-2}'s header bytes and stream
+//      words from LDS (stores are inline asm the compiler's waits do not see);
+//   2. loads the first lookback window of t_{i-1} (whose byte count and those
+//      of its round were published at the end of iteration i-1);
+//   3. runs pass 1 of t_i on data loaded during iteration i-1 and, group by
+//      group, loads t_{i+1} into the same registers;
+//   4. publishes t_i's byte count and parks its words and metadata in LDS.
+// Every wait of iteration i is for loads issued about one pass 1 earlier, so
+// no memory latency is exposed while the lookback has a whole iteration to
+// see its predecessors. The last tiles of an item (per workgroup) drain the
+// pipeline in extra iterations without a pass 1.
 template <int DIM, int MODE, class T, int G, int WAVES, int GMAX, class Src>
 __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint64_t& tile, const uint64_t ntiles,
-                                             const ScanCtx& sc, Smem<WAVES, GMAX>& sm) {
+                                             const ScanCtx& sc, SmemC<WAVES, GMAX>& sm) {
     constexpr int S = (DIM == 1) ? 4 : 8;        // bytes per plane
     constexpr int BPG = (DIM == 1) ? 2 : 1;      // blocks per group
     constexpr int NB = BPG * G;                  // blocks per wave
+    constexpr int KLB = (16 / WAVES > 1) ? 16 / WAVES : 1;   // lookback words per lane
     static_assert(NB <= kWaveSize, "one bookkeeping lane per block");
     static_assert(G <= GMAX, "shared memory sized for the largest variant");
 
@@ -820,16 +902,12 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     // after the transpose, element 0 of a block is bit 7 of its plane words: in
     // both halves for 1D (two blocks), only in the low half for 2D/3D
     const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
-    const uint64_t stride = gridDim.x;
+    const uint32_t stride = gridDim.x;
 
     Item cur = src.get(it);
     LaneGeo<DIM> lg;
     lg.init(lane, cur.geo);
 
-    // Loads are software-pipelined across tiles: as soon as pass 1 has consumed
-    // group g of the current tile, the same register receives group g of the
-    // next tile of this item, so every wave keeps its loads in flight through
-    // pass 1, the scan and the stores.
     T xv[G];
     uint64_t vmask = 0;   // bit g: this lane's element of group g lies inside the array
     auto load_tile = [&](uint64_t lt) {
@@ -848,26 +926,81 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
             if constexpr (DIM != 1) wk.next(geo);
         }
     };
-    load_tile(tile - cur.tile0);
 
+    // resolve the lookback of tile t (slot s, ttot bytes) and write its stream (every wave)
+    uint64_t lbw[KLB];
+    auto finish = [&](uint32_t s, uint64_t t, uint64_t ttot) {
+        const Geom& geo = cur.geo;
+        const T r = (T)cur.param;
+        const uint64_t excl = wg_lookback<WAVES, KLB>(sc, t, cur.tile0, ttot, wave, lane, sm.lb,
+            [&](uint64_t p) { return compress_tile_bytes<DIM, MODE, T, G, WAVES>((const T*)cur.src, geo, r, p - cur.tile0, lane); },
+            true, &lbw);
+        if (t - cur.tile0 == geo.ntiles - 1 && threadIdx.x == 0 && cur.total) *cur.total = geo.H + excl + ttot;
+        const uint32_t wexcl = sm.wave[s][wave][0], wbytes = sm.wave[s][wave][1], big = sm.wave[s][wave][2];
+        const uint32_t ls0 = lane < NB ? sm.lane[s][wave][lane][0] : 0u;   // bookkeeping lanes only
+        // header bytes (lane j: block j of the wave); 2D/3D pad bytes [nblocks, H) are 0
+        const uint64_t grp0 = ((uint64_t)(t - cur.tile0) * WAVES + (uint64_t)wave) * G;
+        const uint64_t b0 = (DIM == 1) ? 2 * grp0 : grp0;
+        if (lane < NB && (DIM == 1 || b0 + (uint64_t)lane < geo.H))
+            st_async_b8((uint8_t*)cur.dst + b0, (uint32_t)lane, ls0 >> 24);
+        const uint64_t wave_base = excl + wexcl;
+        uint8_t* const wbase = (uint8_t*)cur.dst + uniform64(geo.H + wave_base);
+        const bool fits = geo.H + wave_base + wbytes <= cur.limit;   // never write past the caller's buffer
+        if (!fits && lane == 0) atomicOr(cur.err, kErrOverflow);
+        if (fits) {
+            if constexpr (MODE == kOutlier) {
+                const uint32_t pref = (ls0 >> 20) & 7;
+                if (lane < NB && pref) st_async_le(wbase, ls0 & 0xfffffu, sm.lane[s][wave][lane][1], pref);
+            }
+            static_for<0, G>([&](auto gc) {
+                constexpr int g = decltype(gc)::value;
+                const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
+                const bool wideg = (DIM != 1) && ((big >> g) & 1);
+                uint32_t w = sm.words[s][wave][g][lane];
+                if constexpr (MODE == kOutlier) {
+                    if (m.sel() && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
+                }
+                if (m.has(l)) st_async_b32(wbase, m.off() + (wideg ? wo_wide : wo), w);
+                if constexpr (DIM != 1) {
+                    if (wideg && m.nz() && lane < 2)
+                        st_async_b32(wbase, m.off() + 4 * lane, sm.aux[s][wave][2 * g + lane]);
+                }
+            });
+        }
+    };
+
+    for (;;) {   // items of this variant
+    load_tile(tile - cur.tile0);
+    // pipeline: npend tiles (0..2) wait for their stores; slot is where the
+    // tile entering pass 1 parks; the pending ones are tile - stride (slot - 1)
+    // and tile - 2 * stride (slot - 2), with byte counts ttot1 and ttot2
+    uint32_t npend = 0, slot = 0;
+    uint64_t ttot1 = 0, ttot2 = 0;
     for (;;) {
     const Geom& geo = cur.geo;
+    const uint32_t s2 = (slot + 1) % 3;   // t_{i-2}
+    const uint32_t s1 = (slot + 2) % 3;   // t_{i-1}
+    // 1. oldest pending tile: lookback + stores
+    if (npend == 2) finish(s2, tile - 2 * (uint64_t)stride, ttot2);
+    // 2. first lookback window of the younger pending tile
+    if (npend >= 1) lookback_issue<WAVES, KLB>(sc, tile - stride, cur.tile0, wave, lane, lbw);
+
+    // 3. pass 1 of t_i
     const uint64_t lt = tile - cur.tile0;
     const T r = (T)cur.param;
     const uint64_t grp0 = (lt * WAVES + (uint64_t)wave) * G;
-
-    uint32_t W[G];
-    uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
-    uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
     const uint64_t next = tile + stride;
     const bool same = next < cur.tile0 + geo.ntiles;   // next tile in this item
     const T* __restrict__ in = (const T*)cur.src;
     const uint64_t gn0 = ((next - cur.tile0) * WAVES + (uint64_t)wave) * G;
     // The next tile's groups load during pass 1 when all of this wave's groups
     // of it lie inside the array (then no per-element checks are needed);
-    // otherwise the tile loads after pass 1 with checks.
+    // otherwise the tile loads after pass 1 with checks. The loads are issued
+    // on every iteration (a fixed number per iteration lets the compiler wait
+    // for exactly the loads it needs); without a next tile to roll in, every
+    // lane re-reads element 0.
     bool roll = same;
-    uint64_t nbase = 0;   // element index of the wave's first element of the next tile (2D/3D: block origin)
+    uint64_t nbase = 0;
     if constexpr (DIM == 1) {
         roll = roll && (gn0 + G) * 64 <= geo.n;
         nbase = gn0 * 64;
@@ -878,6 +1011,11 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     if constexpr (DIM != 1) {
         if (roll) wkn.init(gn0, geo);
     }
+    const uint32_t lofs = roll ? (uint32_t)lane : 0u;
+
+    uint32_t W[G];
+    uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
+    uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
     // Tiles entirely inside the array skip the per-element validity select
     // (fixed mode never needs it: x = 0 gives q = 0). 1D instantiates pass 1
     // twice for that; 2D/3D branch per group (two copies cost more registers).
@@ -887,13 +1025,11 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     static_for<0, G>([&](auto gc) {
         constexpr int g = decltype(gc)::value;
         const uint32_t q = quantize<T>(xv[g], r);   // x = 0 outside the array gives q = 0
-        if (roll) {
-            if constexpr (DIM == 1) {
-                xv[g] = __builtin_nontemporal_load(in + nbase + 64 * g + lane);
-            } else {
-                xv[g] = __builtin_nontemporal_load(in + block_origin<DIM>(geo, wkn) + lg.off);
-                wkn.next(geo);
-            }
+        if constexpr (DIM == 1) {
+            xv[g] = __builtin_nontemporal_load(in + (roll ? nbase + 64 * g : 0) + lofs);
+        } else {
+            xv[g] = __builtin_nontemporal_load(in + (roll ? block_origin<DIM>(geo, wkn) + lg.off : 0));
+            if (roll) wkn.next(geo);
         }
         uint32_t d;
         if constexpr (MODE == kFixed) {
@@ -929,8 +1065,8 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
             if constexpr (MODE == kOutlier) m2 = ballot((w & not_e0) != 0);
             const uint64_t sg = ballot((int32_t)dm < 0);
             if (lane == 0) {
-                sm.aux[wave][2 * g] = (uint32_t)sg;
-                sm.aux[wave][2 * g + 1] = (uint32_t)(sg >> 32);
+                sm.aux[slot][wave][2 * g] = (uint32_t)sg;
+                sm.aux[slot][wave][2 * g + 1] = (uint32_t)(sg >> 32);
             }
             big |= 1u << g;
         }
@@ -955,11 +1091,9 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     if (DIM == 1 && full) pass1(std::true_type{});
     else pass1(std::false_type{});
     if (roll) vmask = (G == 64) ? ~0ull : (1ull << G) - 1;
-    else if (same) load_tile(next - cur.tile0);   // in flight during scan + stores
+    else if (same) load_tile(next - cur.tile0);   // a partial next tile: loads with checks
 
     // ---- bookkeeping: lane j <-> block j of this wave -----------------------
-    uint8_t* __restrict__ out = (uint8_t*)cur.dst;
-    const uint64_t blk_idx = (DIM == 1) ? 2 * grp0 + (uint64_t)lane : grp0 + (uint64_t)lane;
     const bool blk = lane < NB;
     uint32_t byte;
     if constexpr (MODE == kOutlier) byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
@@ -970,57 +1104,76 @@ __device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint6
     const uint32_t off = incl - sz;
     const uint64_t wtot = rdlane(incl, kWaveSize - 1);
     if (blk) {
-        sm.meta[wave][lane] = make_meta(off, v, sz != 0);
-        // header byte; 2D/3D pad bytes [nblocks, H) are written as 0
-        if (DIM == 1 || blk_idx < geo.H) out[blk_idx] = (uint8_t)byte;
+        sm.meta[slot][wave][lane] = make_meta(off, v, sz != 0);
+        sm.lane[slot][wave][lane][0] = off | (((blk && v.sel) ? v.nb : 0u) << 20) | (byte << 24);
+        sm.lane[slot][wave][lane][1] = bko;
     }
-
-    const TileOffsets to = tile_scan<WAVES, (8 / WAVES > 1 ? 8 / WAVES : 1)>(sc, tile, cur.tile0, wtot, wave, lane, sm.wtot, sm.lb,
-        [&](uint64_t p) { return compress_tile_bytes<DIM, MODE, T, G, WAVES>((const T*)cur.src, geo, r, p - cur.tile0, lane); });
-    if (lt == geo.ntiles - 1 && threadIdx.x == 0 && cur.total) *cur.total = geo.H + to.tile_incl;
-    uint8_t* const wbase = out + uniform64(geo.H + to.wave_base);
-    const bool fits = geo.H + to.wave_base + wtot <= cur.limit;   // never write past the caller's buffer
-    if (!fits && lane == 0) atomicOr(cur.err, kErrOverflow);
-    if (fits) {
-        if constexpr (MODE == kOutlier) {
-            if (blk && v.sel) store_le(wbase + off, bko, v.nb);   // little-endian |d| of element 0
-        }
-        // ---- pass 2: stream words --------------------------------------------
-        static_for<0, G>([&](auto gc) {
-            constexpr int g = decltype(gc)::value;
-            const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
-            const bool wideg = (DIM != 1) && ((big >> g) & 1);
-            uint32_t w = W[g];
-            if constexpr (MODE == kOutlier) {
-                if (m.sel() && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
-            }
-            if (m.has(l)) store_u32u(wbase + (m.off() + (wideg ? wo_wide : wo)), w);
-            if constexpr (DIM != 1) {
-                if (wideg && m.nz() && lane < 2) store_u32u(wbase + (m.off() + 4 * lane), sm.aux[wave][2 * g + lane]);
-            }
-        });
+#pragma unroll
+    for (int g = 0; g < G; ++g) sm.words[slot][wave][g][lane] = W[g];
+    // combine the wave totals and publish the tile's byte count right away
+    if (lane == 0) sm.wtot[slot][wave] = wtot;
+    __syncthreads();
+    uint64_t wexcl = 0, ttot = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+        const uint64_t t = sm.wtot[slot][w];
+        wexcl += (w < wave) ? t : 0;
+        ttot += t;
     }
+    if (wave == 0 && lane == 0)
+        st_status_async(sc.status, (uint32_t)tile, sc.tag | (tile == cur.tile0 ? kFlagPrefix : kFlagAgg) | ttot);
+    if (lane == 0) {   // read back by this wave only
+        sm.wave[slot][wave][0] = (uint32_t)wexcl;
+        sm.wave[slot][wave][1] = (uint32_t)wtot;
+        sm.wave[slot][wave][2] = big;
+    }
+    npend = npend == 2 ? 2 : npend + 1;
+    slot = (slot + 1) % 3;
+    ttot2 = ttot1;
+    ttot1 = ttot;
 
     tile = next;
-    if (same) continue;
+    if (!same) break;
+    }   // tiles of this item
+    // drain: the last one or two tiles of this item for this workgroup
+    {
+        const uint64_t last = tile - stride;
+        if (npend == 2) finish((slot + 1) % 3, last - stride, ttot2);
+        lookback_issue<WAVES, KLB>(sc, last, cur.tile0, wave, lane, lbw);
+        finish((slot + 2) % 3, last, ttot1);
+    }
     if (tile >= ntiles) return;
     it = src.locate(tile, it);
     if (src.variant(it) != cur.variant) return;
     cur = src.get(it);
     lg.init(lane, cur.geo);
-    load_tile(tile - cur.tile0);
-    }   // tiles
+    }   // items
 }
+
 
 // ---------------------------------------------------------------------------
 // decompress: persistent loop, same contract as compress_run
 // ---------------------------------------------------------------------------
+// Shared memory of a decompress workgroup, sized for the largest variant.
+template <int WAVES, int GMAX>
+struct SmemD {
+    LookbackShared<WAVES> lb;
+    uint64_t wtot[2][WAVES];
+    Meta meta[2][WAVES][2 * GMAX];    // per-block metadata of the current and the pending tile
+    uint32_t aux[WAVES][2 * GMAX];    // outlier |d| of element 0 per block of the tile being decoded
+};
+
+// Same one-deep pipeline as compress_run: iteration i turns tile t_i's header
+// bytes into block sizes and publishes its byte count at once (its header
+// bytes were loaded during iteration i-1), then resolves the lookback of
+// t_{i-1} and decodes it.
 template <int DIM, int MODE, class T, int G, int WAVES, int GMAX, class Src>
 __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uint64_t& tile, const uint64_t ntiles,
-                                               const ScanCtx& sc, Smem<WAVES, GMAX>& sm) {
+                                               const ScanCtx& sc, SmemD<WAVES, GMAX>& sm) {
     constexpr int S = (DIM == 1) ? 4 : 8;
     constexpr int BPG = (DIM == 1) ? 2 : 1;
     constexpr int NB = BPG * G;
+    constexpr int KLB = (16 / WAVES > 1) ? 16 / WAVES : 1;   // lookback words per lane
     static_assert(G <= GMAX, "shared memory sized for the largest variant");
 
     const int lane = (int)__lane_id();
@@ -1043,11 +1196,82 @@ __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uin
     };
     uint32_t byte = header_byte(tile - cur.tile0);
 
-    for (;;) {
+    // the pending tile: block sizes known and byte count published
+    bool pend = false;
+    uint32_t pbuf = 0;
+    uint64_t p_tile = 0, p_wexcl = 0, p_ttot = 0, p_wtot = 0;
+    uint32_t p_off = 0, p_pref = 0;   // bookkeeping lane: block offset, outlier prefix bytes (0: none)
+
+    // lookback, stream loads, decode and stores of the pending tile (every wave)
+    auto flush = [&]() {
+        const Geom& geo = cur.geo;
+        const uint8_t* __restrict__ cmp = (const uint8_t*)cur.src;
+        T* __restrict__ out = (T*)cur.dst;
+        const T eb = (T)cur.param;
+        const uint64_t grp0 = ((p_tile - cur.tile0) * WAVES + (uint64_t)wave) * G;
+        const uint64_t excl = wg_lookback<WAVES, KLB>(sc, p_tile, cur.tile0, p_ttot, wave, lane, sm.lb,
+            [&](uint64_t p) { return header_tile_bytes<DIM, MODE, G, WAVES>(cmp, geo, p - cur.tile0, lane); },
+            true);
+        const uint64_t wave_base = excl + p_wexcl;
+        const uint8_t* const wbase = cmp + uniform64(geo.H + wave_base);
+        const bool intact = geo.H + wave_base + p_wtot <= cur.limit;
+        if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
+        if constexpr (MODE == kOutlier) {
+            // a selected block starts with nb <= 4 prefix bytes and then a 4-byte sign
+            // word, so one u32 read at the block start stays inside the block
+            uint32_t o = 0;
+            if (p_pref && intact) {
+                o = load_u32u(wbase + p_off);
+                if (p_pref < 4) o &= (1u << (8 * p_pref)) - 1;
+            }
+            if (lane < NB) sm.aux[wave][lane] = o;
+            __builtin_amdgcn_wave_barrier();
+        }
+        // ---- the stream words of all G groups -----------------------------------
+        uint32_t W[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const Meta m = sm.meta[pbuf][wave][BPG * g + (DIM == 1 ? h : 0)];
+            const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
+            W[g] = (intact && m.has(l)) ? load_u32u(wbase + (m.off() + (wideg ? wo_wide : wo))) : 0u;
+        }
+        // ---- decode ---------------------------------------------------------------
+        Walker wk;
+        if constexpr (DIM != 1) wk.init(grp0, geo);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const Meta m = sm.meta[pbuf][wave][BPG * g + (DIM == 1 ? h : 0)];
+            const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
+            const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
+            uint32_t a, neg;
+            if (!wideg) {
+                neg = x >> 31;
+                a = x & 0x7fffffffu;
+            } else {   // 32 planes: the sign words sit in their own two words
+                const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
+                neg = (sw >> l) & 1;
+                a = x;
+            }
+            if constexpr (MODE == kOutlier) {
+                if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
+            }
+            const uint32_t dm = neg ? 0u - a : a;
+            const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
+            uint32_t q;
+            if constexpr (MODE == kFixed) q = d;
+            else q = inverse_lorenzo<DIM>(d, lane);
+            const bool zero_blk = !intact || !m.nz();
+            const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
+            uint64_t idx;
+            const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
+            if (valid) out[idx] = val;
+            if constexpr (DIM != 1) wk.next(geo);
+        }
+        pend = false;
+    };
+
+    for (uint32_t cbuf = 0;; cbuf ^= 1) {
     const Geom& geo = cur.geo;
-    const uint8_t* __restrict__ cmp = (const uint8_t*)cur.src;
-    T* __restrict__ out = (T*)cur.dst;
-    const T eb = (T)cur.param;
     const uint64_t lt = tile - cur.tile0;
     const uint64_t grp0 = (lt * WAVES + (uint64_t)wave) * G;
 
@@ -1060,84 +1284,48 @@ __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uin
     const uint32_t off = incl - sz;
     const uint64_t wtot = rdlane(incl, kWaveSize - 1);
     // every entry is rewritten: blocks past the end get an empty lane mask, so no
-    // load below ever uses an offset left over from an earlier tile or array
-    if (lane < NB) sm.meta[wave][lane] = make_meta(off, v, blk && sz != 0);
-
-    const TileOffsets to = tile_scan<WAVES, (16 / WAVES > 1 ? 16 / WAVES : 1)>(sc, tile, cur.tile0, wtot, wave, lane, sm.wtot, sm.lb,
-        [&](uint64_t p) { return header_tile_bytes<DIM, MODE, G, WAVES>(cmp, geo, p - cur.tile0, lane); });
-    if (lt == geo.ntiles - 1 && threadIdx.x == 0 && cur.total) *cur.total = geo.H + to.tile_incl;
-    const uint8_t* const wbase = cmp + uniform64(geo.H + to.wave_base);
-    const bool intact = geo.H + to.wave_base + wtot <= cur.limit;
-    if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
-    if constexpr (MODE == kOutlier) {
-        // a selected block starts with nb <= 4 prefix bytes and then a 4-byte sign
-        // word, so one u32 read at the block start stays inside the block
-        uint32_t o = 0;
-        if (blk && intact && v.sel) {
-            o = load_u32u(wbase + off);
-            if (v.nb < 4) o &= (1u << (8 * v.nb)) - 1;
-        }
-        if (lane < NB) sm.aux[wave][lane] = o;
-        __builtin_amdgcn_wave_barrier();
+    // load ever uses an offset left over from an earlier tile or array
+    if (lane < NB) sm.meta[cbuf][wave][lane] = make_meta(off, v, blk && sz != 0);
+    if (lane == 0) sm.wtot[cbuf][wave] = wtot;
+    __syncthreads();
+    uint64_t wexcl = 0, ttot = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+        const uint64_t t = sm.wtot[cbuf][w];
+        wexcl += (w < wave) ? t : 0;
+        ttot += t;
     }
+    if (wave == 0 && lane == 0) publish_tile(sc, tile, cur.tile0, ttot);
 
-    // the next tile's header bytes load while this tile decodes
+    // the next tile's header bytes load while the pending tile decodes
     const uint64_t next = tile + stride;
     const bool same = next < cur.tile0 + geo.ntiles;
+    const uint32_t c_pref = (blk && v.sel) ? v.nb : 0u;
+    if (same) byte = header_byte(next - cur.tile0);
 
-    // ---- prefetch the stream words of all G groups ---------------------------
-    uint32_t W[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
-        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-        W[g] = (intact && m.has(l)) ? load_u32u(wbase + (m.off() + (wideg ? wo_wide : wo))) : 0u;
-    }
+    if (pend) flush();   // the previous tile's lookback and decode
 
-    // ---- decode ---------------------------------------------------------------
-    Walker wk;
-    if constexpr (DIM != 1) wk.init(grp0, geo);
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
-        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-        const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
-        uint32_t a, neg;
-        if (!wideg) {
-            neg = x >> 31;
-            a = x & 0x7fffffffu;
-        } else {   // 32 planes: the sign words sit in their own two words
-            const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
-            neg = (sw >> l) & 1;
-            a = x;
-        }
-        if constexpr (MODE == kOutlier) {
-            if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
-        }
-        const uint32_t dm = neg ? 0u - a : a;
-        const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
-        uint32_t q;
-        if constexpr (MODE == kFixed) q = d;
-        else q = inverse_lorenzo<DIM>(d, lane);
-        const bool zero_blk = !intact || !m.nz();
-        const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
-        uint64_t idx;
-        const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
-        if (valid) out[idx] = val;
-        if constexpr (DIM != 1) wk.next(geo);
-    }
+    pend = true;
+    pbuf = cbuf;
+    p_tile = tile;
+    p_wexcl = wexcl;
+    p_ttot = ttot;
+    p_wtot = wtot;
+    p_off = off;
+    p_pref = c_pref;
 
     tile = next;
+    if (same) continue;
+    flush();   // last tile of this item for this workgroup
     if (tile >= ntiles) return;
-    if (!same) {
     it = src.locate(tile, it);
     if (src.variant(it) != cur.variant) return;
     cur = src.get(it);
     lg.init(lane, cur.geo);
-    }
     byte = header_byte(tile - cur.tile0);
     }   // tiles
 }
+
 
 // ---------------------------------------------------------------------------
 // single-array kernels: one item, every tile of the same variant
@@ -1145,7 +1333,7 @@ __device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uin
 template <int DIM, int MODE, class T, int G, int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize)
 compress_kernel(const Item item, const ScanCtx sc) {
-    __shared__ Smem<WAVES, G> sm;
+    __shared__ SmemC<WAVES, G> sm;
     const OneItem src{item};
     uint32_t it = 0;
     uint64_t tile = blockIdx.x;
@@ -1155,7 +1343,7 @@ compress_kernel(const Item item, const ScanCtx sc) {
 template <int DIM, int MODE, class T, int G, int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize)
 decompress_kernel(const Item item, const ScanCtx sc) {
-    __shared__ Smem<WAVES, G> sm;
+    __shared__ SmemD<WAVES, G> sm;
     const OneItem src{item};
     uint32_t it = 0;
     uint64_t tile = blockIdx.x;
@@ -1202,7 +1390,7 @@ template <int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize)
 compress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, uint64_t ntiles, ScanCtx sc,
                       const unsigned long long* __restrict__ epoch) {
-    __shared__ Smem<WAVES, kBatchGF32> sm;
+    __shared__ SmemC<WAVES, kBatchGF32> sm;
     sc.tag = (epoch[0] & kEpochMask) << kEpochShift;
     const ItemTable src{items, nitems};
     uint64_t tile = blockIdx.x;
@@ -1219,7 +1407,7 @@ template <int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize)
 decompress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, uint64_t ntiles, ScanCtx sc,
                         const unsigned long long* __restrict__ epoch) {
-    __shared__ Smem<WAVES, kBatchGF32> sm;
+    __shared__ SmemD<WAVES, kBatchGF32> sm;
     sc.tag = (epoch[0] & kEpochMask) << kEpochShift;
     const ItemTable src{items, nitems};
     uint64_t tile = blockIdx.x;

@@ -9,8 +9,10 @@
 
 namespace cuszp {
 
-// Groups (64-element units) per wave; a tile is waves * kGroupsPerWave groups.
-constexpr int kGroupsPerWave = 16;
+// Groups (64-element units) per wave; a tile is waves * groups_per_wave(waves)
+// groups = 4096 elements, so the double-buffered stream words of a compress
+// tile take 32 KB of LDS whatever the workgroup size.
+constexpr int groups_per_wave(int waves) { return waves >= 8 ? 8 : 16; }
 // Batched kernels: groups per wave by element type (f64 tiles hold the same
 // bytes as f32 tiles, and keep the f64 variants' registers in check).
 constexpr int kBatchGroupsF32 = 16;
