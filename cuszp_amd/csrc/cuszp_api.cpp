@@ -116,10 +116,12 @@ int finish_blocking(Slot& s, hipStream_t st, size_t* total) {
 }  // namespace
 
 int& grid_sixteenths(int dev) {
-    static int q[64] = {16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
-                        16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
-                        16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16,
-                        16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16, 16};
+    // 15/16 of the occupancy estimate: on MI355X a full-occupancy grid has been
+    // seen one workgroup slot short (some predecessor then waits for a steal)
+    static int q[64] = {15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15,
+                        15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15,
+                        15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15,
+                        15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15};
     return q[dev & 63];
 }
 
@@ -131,7 +133,7 @@ TilePlan plan_tiles(int dim, const Geom& g) {
     }();
     const uint64_t groups = (dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;   // 64-element units
     int waves = forced;
-    if (!waves) waves = 4;
+    if (!waves) waves = 8;   // 8 waves x 8 groups: two workgroups per CU (LDS), best measured
     const uint64_t per_tile = (uint64_t)waves * groups_per_wave(waves);
     return TilePlan{waves, (groups + per_tile - 1) / per_tile};
 }
