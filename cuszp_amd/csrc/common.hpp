@@ -22,17 +22,6 @@ inline int status_from_bits(unsigned long long bits) {
     return CUSZP_SUCCESS;
 }
 
-// A launch in which some workgroup had to compute a predecessor's byte count
-// ran a larger persistent grid than the device kept resident: use a smaller
-// one from now on (the launch itself completed correctly).
-inline void note_steal(unsigned long long bits) {
-    if (!(bits & kInfoStolen)) return;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return;
-    int& q = grid_sixteenths(dev);
-    if (q > 4) --q;
-}
-
 // Geometry of a (dim, nbEle, dims) triple; mirrors the reference's block counts.
 inline int make_geom(int dim, size_t n, cuszp_dims_t d, Geom& g) {
     std::memset(&g, 0, sizeof(g));

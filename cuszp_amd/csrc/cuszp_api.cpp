@@ -70,13 +70,15 @@ int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
         uint64_t cap = s.cap_tiles * 2;
         if (cap < ntiles) cap = ntiles;
         if (cap < 4096) cap = 4096;
-        CUSZP_HIP(hipMalloc((void**)&s.d_status, cap * sizeof(unsigned long long)));
-        CUSZP_HIP(hipMemsetAsync(s.d_status, 0, cap * sizeof(unsigned long long), st));
+        // kStatusGuard words in front of the status words (lookback windows of
+        // the first tiles read there)
+        CUSZP_HIP(hipMalloc((void**)&s.d_status, (kStatusGuard + cap) * sizeof(unsigned long long)));
+        CUSZP_HIP(hipMemsetAsync(s.d_status, 0, (kStatusGuard + cap) * sizeof(unsigned long long), st));
         s.cap_tiles = cap;
         s.epoch = 0;
     }
     if (++s.epoch >= (1u << 24)) {   // 24-bit epoch tag wrapped: clear stale words
-        CUSZP_HIP(hipMemsetAsync(s.d_status, 0, s.cap_tiles * sizeof(unsigned long long), st));
+        CUSZP_HIP(hipMemsetAsync(s.d_status, 0, (kStatusGuard + s.cap_tiles) * sizeof(unsigned long long), st));
         s.epoch = 1;
     }
     return CUSZP_SUCCESS;
@@ -87,7 +89,7 @@ int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
 // [4..6] watchdog diagnostics.
 ScanCtx scan_ctx(const Slot& s) {
     ScanCtx c;
-    c.status = s.d_status;
+    c.status = s.d_status + kStatusGuard;
     c.err = (unsigned int*)(s.d_misc + 2);
     c.diag = s.d_misc + 4;
     c.tag = (unsigned long long)s.epoch << kEpochShift;
@@ -100,7 +102,6 @@ int finish_blocking(Slot& s, hipStream_t st, size_t* total) {
     CUSZP_HIP(hipStreamSynchronize(st));
     const unsigned long long bits = s.h_res[1] & 0xffffffffull;
     if (total) *total = (size_t)s.h_res[0];
-    note_steal(bits);
     if (bits & kErrStall) {
         CUSZP_HIP(hipMemcpy(s.h_res + 4, s.d_misc + 4, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         std::fprintf(stderr, "cuSZp: internal error: tile %llu stopped waiting for tile %llu (status word 0x%016llx)\n",
@@ -115,17 +116,7 @@ int finish_blocking(Slot& s, hipStream_t st, size_t* total) {
 
 }  // namespace
 
-int& grid_sixteenths(int dev) {
-    // 15/16 of the occupancy estimate: on MI355X a full-occupancy grid has been
-    // seen one workgroup slot short (some predecessor then waits for a steal)
-    static int q[64] = {15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15,
-                        15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15,
-                        15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15,
-                        15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15};
-    return q[dev & 63];
-}
-
-TilePlan plan_tiles(int dim, const Geom& g) {
+TilePlan plan_tiles(int dim, int type, bool compress, const Geom& g) {
     static const int forced = [] {
         const char* e = std::getenv("CUSZP_WAVES");
         const int w = e ? std::atoi(e) : 0;
@@ -133,8 +124,9 @@ TilePlan plan_tiles(int dim, const Geom& g) {
     }();
     const uint64_t groups = (dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;   // 64-element units
     int waves = forced;
-    if (!waves) waves = 8;   // 8 waves x 8 groups: two workgroups per CU (LDS), best measured
-    const uint64_t per_tile = (uint64_t)waves * groups_per_wave(waves);
+    if (!waves) waves = 4;
+    const uint64_t per_tile = compress ? compress_tile_groups(type == CUSZP_TYPE_DOUBLE, waves)
+                                       : decompress_tile_groups(waves);
     return TilePlan{waves, (groups + per_tile - 1) / per_tile};
 }
 
@@ -160,7 +152,7 @@ int compress_once(const void* in, unsigned char* out, size_t cap, size_t n, size
     if (!in || !out) return CUSZP_ERROR_INVALID_ARGUMENT;
     if (max_stream_bytes(dim, g) > kMaxStream) return CUSZP_ERROR_TOO_LARGE;
     if (cap < g.H) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
-    const TilePlan plan = plan_tiles(dim, g);
+    const TilePlan plan = plan_tiles(dim, type, true, g);
     g.ntiles = plan.ntiles;
     int dev = 0;
     CUSZP_HIP(hipGetDevice(&dev));
@@ -189,7 +181,7 @@ int decompress_once(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
     if (!out || !cmp) return CUSZP_ERROR_INVALID_ARGUMENT;
     if (max_stream_bytes(dim, g) > kMaxStream) return CUSZP_ERROR_TOO_LARGE;
     if (cmp_size < g.H) return CUSZP_ERROR_CORRUPT_STREAM;
-    const TilePlan plan = plan_tiles(dim, g);
+    const TilePlan plan = plan_tiles(dim, type, false, g);
     g.ntiles = plan.ntiles;
     int dev = 0;
     CUSZP_HIP(hipGetDevice(&dev));
@@ -208,37 +200,15 @@ int decompress_once(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
     return finish_blocking(s, st, nullptr);
 }
 
-// A stalled scan (the watchdog gave up; not expected while predecessors can be
-// recomputed by waiting waves) is retried once on a quarter-size grid (the
-// inputs are untouched by a stalled launch).
-bool shrink_after_stall(int rc) {
-    if (rc != CUSZP_ERROR_INTERNAL) return false;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return false;
-    int& q = grid_sixteenths(dev);
-    if (q <= 4) return false;
-    q = 4;
-    std::fprintf(stderr, "cuSZp: persistent grid reduced to a quarter of the occupancy estimate\n");
-    return true;
-}
-
 int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
                   unsigned long long* d_size, double eb, int dim, cuszp_dims_t dims, int type, int mode,
                   hipStream_t st, bool blocking) {
-    int rc;
-    do {
-        rc = compress_once(in, out, cap, n, cmp_size, d_size, eb, dim, dims, type, mode, st, blocking);
-    } while (blocking && shrink_after_stall(rc));
-    return rc;
+    return compress_once(in, out, cap, n, cmp_size, d_size, eb, dim, dims, type, mode, st, blocking);
 }
 
 int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_size, double eb, int dim,
                     cuszp_dims_t dims, int type, int mode, hipStream_t st, bool blocking) {
-    int rc;
-    do {
-        rc = decompress_once(out, cmp, n, cmp_size, eb, dim, dims, type, mode, st, blocking);
-    } while (blocking && shrink_after_stall(rc));
-    return rc;
+    return decompress_once(out, cmp, n, cmp_size, eb, dim, dims, type, mode, st, blocking);
 }
 
 void report(const char* fn, int rc) {

@@ -25,7 +25,6 @@ struct cuszp_batch_plan_s {
     int dir = CUSZP_BATCH_COMPRESS;
     int flags = 0;
     int waves = 4;
-    unsigned grid = 0;   // 0: persistent grid from the occupancy model
     std::vector<cuszp_batch_item_t> src;
     std::vector<Item> items;      // non-empty arrays, in caller order
     uint64_t ntiles = 0;
@@ -48,8 +47,6 @@ namespace {
         hipError_t e_ = (call);                        \
         if (e_ != hipSuccess) return CUSZP_ERROR_HIP;  \
     } while (0)
-
-int groups_per_wave(int type) { return type == CUSZP_TYPE_FLOAT ? kBatchGF32 : kBatchGF64; }
 
 int check_item(const cuszp_batch_item_t& it, int dir, Geom& g) {
     const int dim = it.dim, type = it.type, mode = it.mode;
@@ -74,7 +71,9 @@ int build_table(cuszp_batch_plan_s& p, hipStream_t st) {
         Geom g;
         make_geom(s.dim, s.nbEle, s.dims, g);
         const uint64_t groups = (s.dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;
-        const uint64_t per_tile = (uint64_t)p.waves * groups_per_wave(s.type);
+        const uint64_t per_tile = p.dir == CUSZP_BATCH_COMPRESS
+                                      ? compress_tile_groups(s.type == CUSZP_TYPE_DOUBLE, p.waves)
+                                      : decompress_tile_groups(p.waves);
         g.ntiles = (groups + per_tile - 1) / per_tile;
         const bool c = p.dir == CUSZP_BATCH_COMPRESS;
         Item it = make_item(s.d_in, s.d_out, s.capacity, c ? p.d_sizes + i : nullptr, p.d_err + i,
@@ -94,12 +93,13 @@ int build_table(cuszp_batch_plan_s& p, hipStream_t st) {
     if (p.ntiles > p.cap_tiles) {
         if (p.d_status) BATCH_HIP(hipFree(p.d_status));
         p.d_status = nullptr;
-        BATCH_HIP(hipMalloc((void**)&p.d_status, p.ntiles * sizeof(unsigned long long)));
+        // kStatusGuard words in front (lookback windows of first tiles read there)
+        BATCH_HIP(hipMalloc((void**)&p.d_status, (kStatusGuard + p.ntiles) * sizeof(unsigned long long)));
         p.cap_tiles = p.ntiles;
     }
     // stale status words must not carry the current epoch: start from zero and
     // restart the epoch whenever the table changes
-    BATCH_HIP(hipMemsetAsync(p.d_status, 0, p.cap_tiles * sizeof(unsigned long long), st));
+    BATCH_HIP(hipMemsetAsync(p.d_status, 0, (kStatusGuard + p.cap_tiles) * sizeof(unsigned long long), st));
     BATCH_HIP(hipMemsetAsync(p.d_misc, 0, sizeof(unsigned long long), st));
     if (!p.items.empty())
         BATCH_HIP(hipMemcpyAsync(p.d_items, p.items.data(), p.items.size() * sizeof(Item), hipMemcpyHostToDevice, st));
@@ -110,16 +110,16 @@ int build_table(cuszp_batch_plan_s& p, hipStream_t st) {
 int enqueue(const cuszp_batch_plan_s& p, hipStream_t st) {
     if (p.items.empty()) return CUSZP_SUCCESS;
     ScanCtx sc;
-    sc.status = p.d_status;
+    sc.status = p.d_status + kStatusGuard;
     sc.err = (unsigned int*)(p.d_misc + 2);
     sc.diag = p.d_misc + 4;
     sc.tag = 0;   // the kernels read the epoch word
     BATCH_HIP(launch_bump_epoch(p.d_misc, st));
     const uint32_t n = (uint32_t)p.items.size();
     if (p.dir == CUSZP_BATCH_COMPRESS)
-        BATCH_HIP(launch_compress_batch(p.waves, p.grid, p.d_items, n, p.ntiles, sc, p.d_misc, st));
+        BATCH_HIP(launch_compress_batch(p.waves, p.d_items, n, p.ntiles, sc, p.d_misc, st));
     else
-        BATCH_HIP(launch_decompress_batch(p.waves, p.grid, p.d_items, n, p.ntiles, sc, p.d_misc, st));
+        BATCH_HIP(launch_decompress_batch(p.waves, p.d_items, n, p.ntiles, sc, p.d_misc, st));
     return CUSZP_SUCCESS;
 }
 
@@ -130,11 +130,8 @@ void drop_graph(cuszp_batch_plan_s& p) {
     p.graph = nullptr;
 }
 
-unsigned grid_of(const cuszp_batch_plan_s& p, int sixteenths);
-
 int capture(cuszp_batch_plan_s& p) {
     drop_graph(p);
-    if (!p.grid) p.grid = grid_of(p, grid_sixteenths(p.dev));   // no occupancy queries during capture
     if (!p.cap_stream) BATCH_HIP(hipStreamCreateWithFlags(&p.cap_stream, hipStreamNonBlocking));
     BATCH_HIP(hipStreamBeginCapture(p.cap_stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue(p, p.cap_stream);
@@ -154,7 +151,7 @@ int capture(cuszp_batch_plan_s& p) {
 // (log2 of its tile count and of its array count); later plans of the same
 // class start from the recorded shape.
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tuned;   // -> {waves, grid sixteenths}
+std::map<std::tuple<int, int, int>, int> g_tuned;   // -> waves per workgroup
 
 std::tuple<int, int, int> size_class(const cuszp_batch_plan_s& p) {
     uint64_t bytes = 0;
@@ -163,12 +160,6 @@ std::tuple<int, int, int> size_class(const cuszp_batch_plan_s& p) {
     while ((bytes >> lb) > 1) ++lb;
     while ((p.src.size() >> lc) > 1) ++lc;
     return std::make_tuple(p.dir, lb, lc);
-}
-
-unsigned grid_of(const cuszp_batch_plan_s& p, int sixteenths) {
-    const int res = batch_resident(p.dir == CUSZP_BATCH_COMPRESS, p.waves);
-    const unsigned g = (unsigned)std::max(1, res * sixteenths / 16);
-    return g;
 }
 
 }  // namespace
@@ -199,7 +190,7 @@ int cuSZp_batch_plan_create(cuszp_batch_plan_t* out, const cuszp_batch_item_t* i
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto f = g_tuned.find(size_class(*p));
-        if (f != g_tuned.end()) p->waves = f->second.first;
+        if (f != g_tuned.end()) p->waves = f->second;
     }
     if (!rc && hipMalloc((void**)&p->d_misc, 8 * sizeof(unsigned long long)) != hipSuccess) rc = CUSZP_ERROR_HIP;
     const size_t n = count ? count : 1;
@@ -209,11 +200,6 @@ int cuSZp_batch_plan_create(cuszp_batch_plan_t* out, const cuszp_batch_item_t* i
     if (!rc && hipMemsetAsync(p->d_err, 0, n * sizeof(unsigned int), stream) != hipSuccess) rc = CUSZP_ERROR_HIP;
     if (!rc && hipMemsetAsync(p->d_sizes, 0, n * sizeof(unsigned long long), stream) != hipSuccess) rc = CUSZP_ERROR_HIP;
     if (!rc) rc = build_table(*p, stream);
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto f = g_tuned.find(size_class(*p));
-        if (f != g_tuned.end()) p->grid = grid_of(*p, f->second.second);
-    }
     if (!rc && (flags & CUSZP_BATCH_TUNE)) rc = cuSZp_batch_plan_tune(p, stream, nullptr, nullptr);
     if (!rc && (flags & CUSZP_BATCH_GRAPH) && !p->exec) rc = capture(*p);
     if (rc) {
@@ -245,7 +231,6 @@ int cuSZp_batch_plan_results(cuszp_batch_plan_t p, size_t* cmpSizes, int* status
                              hipMemcpyDeviceToHost, stream));
     BATCH_HIP(hipStreamSynchronize(stream));
     const unsigned long long launch_bits = misc[2] & 0xffffffffull;
-    note_steal(launch_bits);
     if (launch_bits & kErrStall)
         std::fprintf(stderr, "cuSZp: internal error: batched tile %llu stopped waiting for tile %llu\n", misc[4] - 1,
                      misc[5]);
@@ -270,37 +255,31 @@ int cuSZp_batch_plan_tune(cuszp_batch_plan_t p, hipStream_t stream, int* waves_o
     if (!p) return CUSZP_ERROR_INVALID_ARGUMENT;
     if (p->items.empty()) return CUSZP_SUCCESS;
     const int kWaves[2] = {4, 8};
-    const int kSixteenths[3] = {16, 12, 8};
     hipEvent_t e0, e1;
     BATCH_HIP(hipEventCreate(&e0));
     BATCH_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_w = p->waves, best_q = 16;
+    int best_w = p->waves;
     int rc = CUSZP_SUCCESS;
     for (int w : kWaves) {
         p->waves = w;
         rc = build_table(*p, stream);
         if (rc) break;
-        for (int q : kSixteenths) {
-            p->grid = grid_of(*p, q);
-            rc = enqueue(*p, stream);   // warm-up
-            if (rc) break;
-            hipEventRecord(e0, stream);
-            for (int r = 0; r < 3 && !rc; ++r) rc = enqueue(*p, stream);
-            hipEventRecord(e1, stream);
-            if (rc || hipEventSynchronize(e1) != hipSuccess) {
-                rc = rc ? rc : CUSZP_ERROR_HIP;
-                break;
-            }
-            float ms = 0;
-            hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) {
-                best = ms;
-                best_w = w;
-                best_q = q;
-            }
-        }
+        rc = enqueue(*p, stream);   // warm-up
         if (rc) break;
+        hipEventRecord(e0, stream);
+        for (int r = 0; r < 3 && !rc; ++r) rc = enqueue(*p, stream);
+        hipEventRecord(e1, stream);
+        if (rc || hipEventSynchronize(e1) != hipSuccess) {
+            rc = rc ? rc : CUSZP_ERROR_HIP;
+            break;
+        }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+            best = ms;
+            best_w = w;
+        }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -308,10 +287,9 @@ int cuSZp_batch_plan_tune(cuszp_batch_plan_t p, hipStream_t stream, int* waves_o
     p->waves = best_w;
     rc = build_table(*p, stream);
     if (rc) return rc;
-    p->grid = grid_of(*p, best_q);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned[size_class(*p)] = {best_w, best_q};
+        g_tuned[size_class(*p)] = best_w;
     }
     if (p->exec || (p->flags & CUSZP_BATCH_GRAPH)) rc = capture(*p);
     // tuning runs leave per-array state from the candidate launches: clear it
@@ -320,7 +298,7 @@ int cuSZp_batch_plan_tune(cuszp_batch_plan_t p, hipStream_t stream, int* waves_o
     if (!rc && hipMemsetAsync(p->d_misc + 2, 0, 6 * sizeof(unsigned long long), stream) != hipSuccess)
         rc = CUSZP_ERROR_HIP;
     if (waves_out) *waves_out = best_w;
-    if (grid_out) *grid_out = p->grid;
+    if (grid_out) *grid_out = (unsigned)(p->ntiles < kMaxGrid ? p->ntiles : kMaxGrid);
     return rc;
 }
 

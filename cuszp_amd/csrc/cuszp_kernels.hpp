@@ -282,6 +282,9 @@ __device__ __forceinline__ void store_le(uint8_t* p, uint32_t v, uint32_t nb) {
         if (nb & 1) { *p = (uint8_t)v; }
     }
 }
+// LDS byte pointers and unaligned LDS words (gfx950 accesses LDS unaligned).
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef __attribute__((address_space(3))) uint32_t __attribute__((aligned(1))) lds_u32u;
 __device__ __forceinline__ void store_u32u(uint8_t* p, uint32_t v) {
     typedef uint32_t __attribute__((aligned(1))) u32u;
     *(u32u*)p = v;
@@ -296,6 +299,14 @@ __device__ __forceinline__ uint32_t load_u32u(const uint8_t* p) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t ld_status(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// A possibly stale read (may be served by this XCD's L2). Status words only
+// move forward within a launch (nothing -> aggregate -> inclusive prefix; the
+// values never change) and carry the launch's epoch, so a stale word is still
+// a true statement about its tile; words that read as not ready are re-read
+// with ld_status.
+__device__ __forceinline__ uint64_t ld_status_cached(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void st_status(unsigned long long* p, uint64_t v) {
     __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -323,252 +334,130 @@ __device__ __forceinline__ bool status_ready(uint64_t w, uint64_t tag) {
     return ((w & kTagMask) == tag) && f != 0 && f != kFlagClaim;
 }
 
-// Lookback state shared by the waves of a workgroup.
-template <int WAVES>
-struct LookbackShared {
-    uint32_t pos[2][WAVES][2];   // per poll round (double-buffered): nearest prefix, nearest pending position
-    uint32_t ctl[2];             // wave 0's decisions per poll round: bit 0 steal, bit 1 give up
-    uint64_t part[WAVES];        // per-wave sums of a resolved window
-};
-
 // Exclusive prefix of `tile` within its segment (tiles first .. tile-1), run by
-// every wave of the workgroup (uniform control flow across the workgroup).
-// `published`: the caller already published the tile's aggregate (or, for the
-// segment's first tile, its inclusive prefix) with publish_tile().
+// one whole wave (no workgroup barriers inside).
 //
-// The workgroup publishes the tile aggregate, then reads predecessors a window
-// of WAVES * 64 * K at a time: wave v, lane j, word k reads position
-// p = 64*(K*v + k) + j, i.e. tile - 1 - p. A window is resolved once every
-// position up to the nearest published inclusive prefix P is ready; the sum of
-// the values at positions <= P is the exclusive prefix (or, with no prefix in
-// the window, its sum carries into the next window). The inclusive prefix is
-// then published. In a persistent grid of B workgroups the nearest prefix
-// trails a tile by up to about B positions, so the window is sized to cover B
-// in one round trip of status loads.
+// The wave publishes the tile's aggregate (the segment's first tile publishes
+// its inclusive prefix and returns 0), then reads predecessors a window of
+// 64 * K at a time: lane j, word k reads position p = 64k + j, i.e. tile
+// tile - 1 - p (the K loads of a window are issued back to back: one round
+// trip). A window is resolved once every position up to the nearest published
+// inclusive prefix P is ready; the values at positions <= P sum to the
+// exclusive prefix (with no prefix in the window, the window's sum carries
+// into the next one). The wave then publishes the tile's inclusive prefix.
+// Positions before the segment read words of an earlier segment or of the
+// guard in front of the status array (kStatusGuard words) and count as an
+// inclusive prefix of 0.
 //
-// Forward progress does not depend on which workgroups are resident: when a
-// window stays unresolved for kStealTicks, each wave claims its farthest
-// pending position (CAS from the stale word to CLAIM), computes that tile's
-// aggregate with agg_of(tile) and publishes it. A claim is held only by a
-// running wave that computes without waiting, so every wait ends; the owner's
-// later store of the same aggregate (or of its prefix) is harmless.
-template <int WAVES, int K, class AggF>
-__device__ __forceinline__ uint64_t wg_lookback(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t agg,
-                                                int wave, int lane, LookbackShared<WAVES>& ls, AggF&& agg_of,
-                                                bool published = false, const uint64_t (*pre)[K] = nullptr) {
-    constexpr int SPAN = 64 * K;           // positions per wave and window
-    constexpr int WIN = WAVES * SPAN;      // positions per window
+// Tiles are numbered in launch order, so a predecessor normally belongs to a
+// workgroup that started earlier. Forward progress does not depend on that:
+// when a window stays unresolved for kStealTicks, the wave claims the farthest
+// pending predecessor of the window (CAS from the word it saw to CLAIM),
+// computes that tile's aggregate with agg_of(tile) and publishes it. A claim is
+// held only by a running wave that computes without waiting, so every wait
+// ends; the owner's later store of the same aggregate (or of its inclusive
+// prefix) is harmless.
+template <int K, class AggF>
+__device__ __forceinline__ uint64_t wave_lookback(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t agg,
+                                                  int lane, AggF&& agg_of) {
+    constexpr int WIN = 64 * K;
     const uint64_t tag = sc.tag;
-    const bool leader = (wave == 0) && (lane == 0);
     if (tile == first) {
-        if (leader && !published) st_status(sc.status + tile, tag | kFlagPrefix | agg);
+        if (lane == 0) st_status(sc.status + tile, tag | kFlagPrefix | agg);
         return 0;
     }
-    if (leader && !published) st_status(sc.status + tile, tag | kFlagAgg | agg);
-    const int pbase = wave * SPAN;
+    if (lane == 0) st_status(sc.status + tile, tag | kFlagAgg | agg);
     uint64_t excl = 0;
     int64_t hi = (int64_t)tile;
-    uint32_t spins = 0, round = 0;
+    uint32_t spins = 0;
     const uint64_t t_wait = __builtin_amdgcn_s_memrealtime();
-    bool steal = false, stalled = false;
-    bool use_pre = pre != nullptr;
+    bool stalled = false;
+    int64_t claimed = -1;   // a predecessor this wave claimed and must compute
     for (;;) {
-        uint64_t w[K];
-        bool ready[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int64_t idx = hi - 1 - (pbase + 64 * k + lane);
-            w[k] = tag | kFlagPrefix;   // before the segment: inclusive prefix 0
-            ready[k] = true;
-            if (idx >= (int64_t)first) {
-                w[k] = use_pre ? (*pre)[k] : ld_status(sc.status + idx);
-                ready[k] = status_ready(w[k], tag);
+        if (claimed >= 0) {   // computed with no window words live; the window is then re-read
+            const uint64_t a = agg_of((uint64_t)claimed);
+            if (lane == 0) {
+                cas_status(sc.status + claimed, tag | kFlagClaim, tag | kFlagAgg | a);
+                atomicOr(sc.err, kInfoStolen);
             }
+            claimed = -1;
         }
-        use_pre = false;   // only the first window comes pre-loaded
+        const unsigned long long* const wp = sc.status + (hi - 1 - lane);   // position 64k + lane: wp - 64k
+        const int64_t rel = hi - 1 - lane - (int64_t)first;                 // before the segment iff rel < 64k
+        uint64_t w[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) w[k] = ld_status(wp - 64 * k);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (rel < 64 * k) w[k] = tag | kFlagPrefix;
         int P;
         for (;;) {
-            int Pv = WIN, Nv = WIN;
-#pragma unroll
-            for (int k = K - 1; k >= 0; --k) {
-                const uint64_t pm = ballot(ready[k] && (w[k] & kFlagMask) == kFlagPrefix);
-                if (pm) Pv = pbase + 64 * k + __builtin_ctzll(pm);
-                const uint64_t nr = ballot(!ready[k]);
-                if (nr) Nv = pbase + 64 * k + __builtin_ctzll(nr);
-            }
-            const uint32_t par = round & 1;
-            if (lane == 0) {
-                ls.pos[par][wave][0] = (uint32_t)Pv;
-                ls.pos[par][wave][1] = (uint32_t)Nv;
-                if (wave == 0) {
-                    uint32_t c = 0;
-                    if (__builtin_amdgcn_s_memrealtime() - t_wait > kStealTicks) c |= 1;
-                    if (spins + 1 >= kSpinLimit ||
-                        (((spins + 1) & 1023) == 0 &&
-                         (__hip_atomic_load(sc.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kErrStall)))
-                        c |= 2;
-                    ls.ctl[par] = c;
-                }
-            }
-            __syncthreads();
             P = WIN;
             int N = WIN;
+            uint64_t nr[K];
 #pragma unroll
-            for (int v = 0; v < WAVES; ++v) {
-                P = min(P, (int)ls.pos[par][v][0]);
-                N = min(N, (int)ls.pos[par][v][1]);
+            for (int k = K - 1; k >= 0; --k) {
+                const bool rd = status_ready(w[k], tag);
+                const uint64_t pm = ballot(rd && (w[k] & kFlagMask) == kFlagPrefix);
+                if (pm) P = 64 * k + __builtin_ctzll(pm);
+                nr[k] = ballot(!rd);
+                if (nr[k]) N = 64 * k + __builtin_ctzll(nr[k]);
             }
-            const uint32_t ctl = ls.ctl[par];
-            ++round;
-            ++spins;
             if (N > P || N == WIN) break;   // every position up to P is ready
-            // nearest pending position of this wave at or below P
-            int sk = -1, sl = 0;
+            ++spins;
+            // farthest pending position at or below P: (sk, sl)
+            int sk = 0, sl = 0;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                uint64_t nr = ballot(!ready[k]);
-                const int lo = pbase + 64 * k;
-                if (P < lo) nr = 0;
-                else if (P < lo + 63) nr &= (2ull << (P - lo)) - 1;
-                if (nr) { sk = k; sl = 63 - __builtin_clzll(nr); }
+                const int lo = 64 * k;
+                const uint64_t m = P < lo ? 0ull : (P < lo + 63 ? nr[k] & ((2ull << (P - lo)) - 1) : nr[k]);
+                if (m) { sk = k; sl = 63 - __builtin_clzll(m); }
             }
             uint64_t pend = 0;
 #pragma unroll
             for (int k = 0; k < K; ++k) if (k == sk) pend = rdlane64(w[k], sl);
-            const uint64_t pidx = (uint64_t)(hi - 1 - (pbase + 64 * sk + sl));
-            if (ctl & 2) {
+            const int64_t pidx = hi - 1 - (64 * sk + sl);
+            if (spins >= kSpinLimit ||
+                ((spins & 1023) == 0 &&
+                 (__hip_atomic_load(sc.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kErrStall))) {
                 // last-resort watchdog: record the stuck predecessor and give up
                 // (the output is reported invalid; publishing below unblocks successors)
                 stalled = true;
-                if (lane == 0 && sk >= 0) {
+                if (lane == 0) {
                     atomicOr(sc.err, kErrStall);
                     if (atomicCAS(sc.diag, 0ull, tile + 1) == 0ull) {
-                        sc.diag[1] = pidx;
+                        sc.diag[1] = (uint64_t)pidx;
                         sc.diag[2] = pend;
                     }
                 }
                 break;
             }
-            steal |= (ctl & 1) != 0;
-            bool took = false;
-            if (steal && sk >= 0 && !(((pend & kTagMask) == tag) && ((pend & kFlagMask) == kFlagClaim))) {
+            if (__builtin_amdgcn_s_memrealtime() - t_wait > kStealTicks &&
+                !(((pend & kTagMask) == tag) && ((pend & kFlagMask) == kFlagClaim))) {
                 uint64_t seen = 0;
                 if (lane == 0) seen = cas_status(sc.status + pidx, pend, tag | kFlagClaim);
-                seen = rdlane64(seen, 0);
-                if (seen == pend) {
-                    const uint64_t a = agg_of(pidx);
-                    if (lane == 0) {
-                        cas_status(sc.status + pidx, tag | kFlagClaim, tag | kFlagAgg | a);
-                        atomicOr(sc.err, kInfoStolen);
-                    }
-                    took = true;
+                if (rdlane64(seen, 0) == pend) {
+                    claimed = pidx;
+                    break;
                 }
             }
-            if (!took) __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(1);
+            // re-read the pending words (all loads first; a ready word stays ready)
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int64_t idx = hi - 1 - (pbase + 64 * k + lane);
-                if (!ready[k]) {
-                    w[k] = ld_status(sc.status + idx);
-                    ready[k] = status_ready(w[k], tag);
-                }
-            }
+            for (int k = 0; k < K; ++k)
+                if ((nr[k] >> lane) & 1) w[k] = ld_status(wp - 64 * k);
         }
+        if (claimed >= 0) continue;
         uint64_t part = 0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) part += (pbase + 64 * k + lane <= P) ? (w[k] & kValueMask) : 0;
-        part = wave_sum_u40(part);
-        if (lane == 0) ls.part[wave] = part;
-        __syncthreads();
-#pragma unroll
-        for (int v = 0; v < WAVES; ++v) excl += ls.part[v];
+        for (int k = 0; k < K; ++k) part += (64 * k + lane <= P) ? (w[k] & kValueMask) : 0;
+        excl += wave_sum_u40(part);
         if (P < WIN || stalled) break;
         hi -= WIN;
     }
-    if (leader) st_status(sc.status + tile, tag | kFlagPrefix | (excl + agg));
+    if (lane == 0) st_status(sc.status + tile, tag | kFlagPrefix | (excl + agg));
     return excl;
 }
-
-// Early publication of a tile's aggregate (the segment's first tile publishes
-// its inclusive prefix): done as soon as its byte count is known.
-__device__ __forceinline__ void publish_tile(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t agg) {
-    st_status(sc.status + tile, sc.tag | (tile == first ? kFlagPrefix : kFlagAgg) | agg);
-}
-
-// Loads of the first lookback window of `tile` (the words wg_lookback reads
-// at positions p = 64*(K*wave + k) + lane), issued ahead of time: in a
-// pipelined loop they are issued before the next tile's data loads, so waiting
-// for them does not wait for that data. Positions before the segment load the
-// segment's first word instead (wg_lookback replaces them).
-template <int WAVES, int K>
-__device__ __forceinline__ void lookback_issue(const ScanCtx& sc, uint64_t tile, uint64_t first, int wave, int lane,
-                                               uint64_t (&w)[K]) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        int64_t idx = (int64_t)tile - 1 - (wave * 64 * K + 64 * k + lane);
-        if (idx < (int64_t)first) idx = (int64_t)first;
-        w[k] = ld_status(sc.status + idx);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Stores of the hot loops, written as inline asm: the compiler's s_waitcnt
-// insertion does not see them, so its waits for the loop's own loads never
-// wait for these stores to complete (gfx9 counts stores in vmcnt and retires
-// memory operations in order). The loops never read back what they store.
-// Base addresses are wave-uniform (SGPRs), offsets 32-bit per lane.
-//
-// Each starts with s_nop 4: a VMEM instruction that reads an SGPR written by a
-// VALU instruction (here the v_readfirstlane that makes the base uniform) needs
-// 5 wait states on gfx9, and the compiler's hazard recognizer cannot see that
-// an inline asm block is a VMEM instruction.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void st_async_b32(const void* base, uint32_t off, uint32_t v) {
-    asm volatile("s_nop 4\n\tglobal_store_dword %0, %1, %2" : : "v"(off), "v"(v), "s"(uniform64((uint64_t)base)) : "memory");
-}
-__device__ __forceinline__ void st_async_b16(const void* base, uint32_t off, uint32_t v) {
-    asm volatile("s_nop 4\n\tglobal_store_short %0, %1, %2" : : "v"(off), "v"(v), "s"(uniform64((uint64_t)base)) : "memory");
-}
-__device__ __forceinline__ void st_async_b8(const void* base, uint32_t off, uint32_t v) {
-    asm volatile("s_nop 4\n\tglobal_store_byte %0, %1, %2" : : "v"(off), "v"(v), "s"(uniform64((uint64_t)base)) : "memory");
-}
-// little-endian 1..4 bytes
-__device__ __forceinline__ void st_async_le(const void* base, uint32_t off, uint32_t v, uint32_t nb) {
-    if (nb == 4) {
-        st_async_b32(base, off, v);
-    } else {
-        if (nb & 2) { st_async_b16(base, off, v); off += 2; v >>= 16; }
-        if (nb & 1) st_async_b8(base, off, v);
-    }
-}
-// status word publication (relaxed, agent scope: sc1 like __hip_atomic_store)
-__device__ __forceinline__ void st_status_async(const unsigned long long* status, uint32_t tile, uint64_t v) {
-    asm volatile("s_nop 4\n\tglobal_store_dwordx2 %0, %1, %2 sc1" : : "v"(tile * 8u), "v"(v), "s"(uniform64((uint64_t)status)) : "memory");
-}
-
-// Workgroup-level: combine the wave totals and run the lookback; returns
-// {wave's exclusive offset within the stream payload, tile inclusive}.
-struct TileOffsets {
-    uint64_t wave_base;   // payload offset (excluding H) of this wave's first block
-    uint64_t tile_incl;   // payload bytes up to and including this tile
-};
-template <int WAVES, int K, class AggF>
-__device__ __forceinline__ TileOffsets tile_scan(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t wtot,
-                                                 int wave, int lane, uint64_t* s_wtot, LookbackShared<WAVES>& ls,
-                                                 AggF&& agg_of) {
-    if (lane == 0) s_wtot[wave] = wtot;
-    __syncthreads();
-    uint64_t wexcl = 0, ttot = 0;
-#pragma unroll
-    for (int w = 0; w < WAVES; ++w) {
-        const uint64_t v = s_wtot[w];
-        wexcl += (w < wave) ? v : 0;
-        ttot += v;
-    }
-    const uint64_t excl = wg_lookback<WAVES, K>(sc, tile, first, ttot, wave, lane, ls, agg_of);
-    return TileOffsets{excl + wexcl, excl + ttot};
-}
-
 
 // ---------------------------------------------------------------------------
 // Block walking for 2D/3D: wave-uniform block coordinates advanced by +1 block.
@@ -733,16 +622,8 @@ __device__ __forceinline__ uint32_t word_offset(uint32_t l, uint32_t h, bool sig
     }
 }
 
-// Item sources: where a persistent loop finds the array a tile belongs to.
-// OneItem serves the single-array kernels (the descriptor is a kernel
-// argument); ItemTable serves the batched kernels (descriptors in device
-// memory, ordered by tile0).
-struct OneItem {
-    Item item;
-    __device__ __forceinline__ uint32_t locate(uint64_t, uint32_t) const { return 0; }
-    __device__ __forceinline__ Item get(uint32_t) const { return item; }
-    __device__ __forceinline__ uint32_t variant(uint32_t) const { return item.variant; }
-};
+// The arrays of a batched launch: descriptors in device memory, ordered by
+// tile0; a tile belongs to the last array whose tile0 is not above it.
 struct ItemTable {
     const Item* items;
     uint32_t count;
@@ -761,7 +642,7 @@ struct ItemTable {
 };
 
 // ---------------------------------------------------------------------------
-// Aggregates recomputed by a waiting wave (see tile_lookback).
+// Aggregates recomputed by a waiting wave (see wave_lookback).
 // ---------------------------------------------------------------------------
 // OR of v over each 32-lane half (every lane of the half gets the result).
 __device__ __forceinline__ uint32_t or_half(uint32_t v) {
@@ -774,7 +655,7 @@ __device__ __forceinline__ uint32_t or_half(uint32_t v) {
 }
 
 // Payload bytes of local tile `lt` of a compress item, by one wave. Same block
-// sizes as pass 1 of compress_run: a block's rate is the bit length of the OR of
+// sizes as pass 1 of compress_tile: a block's rate is the bit length of the OR of
 // its magnitudes (the highest non-zero plane), 1D ranks |-2^31| as 0.
 template <int DIM, int MODE, class T, int G, int WAVES>
 __device__ uint64_t compress_tile_bytes(const T* __restrict__ in, const Geom& geo, T r, uint64_t lt, int lane) {
@@ -845,578 +726,577 @@ __device__ uint64_t header_tile_bytes(const uint8_t* __restrict__ cmp, const Geo
 }
 
 // ---------------------------------------------------------------------------
-// compress: persistent loop over the tiles of consecutive items of one variant
+// one tile per workgroup pass
 // ---------------------------------------------------------------------------
-// Shared memory of a compress workgroup, sized for the largest variant. Three
-// tiles are in flight per workgroup (see compress_run); everything a tile needs
-// between its pass 1 and its stores lives here, indexed by tile slot.
-template <int WAVES, int GMAX>
+// Lookback window of a tile: 64 * kLookbackWords predecessors per round trip.
+constexpr int kLookbackWords = 4;
+// Words in front of every status array (window reads before tile 0 land there).
+constexpr uint64_t kStatusGuard = 64 * kLookbackWords;
+
+// Tile shapes. A compress wave works through kChunks chunks of ShapeC<T>::G
+// groups (one element per lane per group); a compress tile is WAVES * kChunks
+// * G groups (16384 f32 or 8192 f64 elements at 4 waves). A decompress wave
+// holds G groups at once; a decompress tile is WAVES * G groups.
+template <class T> struct ShapeC;
+template <> struct ShapeC<float> { static constexpr int G = 8; };
+template <> struct ShapeC<double> { static constexpr int G = 4; };
+constexpr int kChunks = 8;
+constexpr int kDecompressG = 16;
+__host__ __device__ constexpr uint64_t compress_tile_groups(bool f64, int waves) {
+    return (uint64_t)waves * kChunks * (f64 ? 4 : 8);
+}
+__host__ __device__ constexpr uint64_t decompress_tile_groups(int waves) { return (uint64_t)waves * kDecompressG; }
+
+// Compressed bytes a compress wave stages in LDS; a wave whose payload is
+// larger (compression ratio below about 2.7 on its share of the tile)
+// recomputes its groups once the tile's offset is known and stores them
+// directly.
+constexpr uint32_t kStageBytes = 6144;
+// Waves per SIMD the compress kernels are built for (the register budget):
+// six 4-wave workgroups of about 26 KB of LDS fit one CU's 160 KB.
+constexpr int kCompressWavesPerSimd = 6;
+
+// Shared memory of a compress workgroup.
+template <int WAVES>
 struct SmemC {
-    LookbackShared<WAVES> lb;
-    uint64_t wtot[3][WAVES];                    // per-wave byte counts
-    Meta meta[3][WAVES][2 * GMAX];              // per-block metadata
-    uint32_t aux[3][WAVES][2 * GMAX];           // sign words of 32-plane 2D/3D groups
-    uint32_t words[3][WAVES][GMAX][kWaveSize];  // stream words
-    uint32_t lane[3][WAVES][2 * GMAX][2];       // per block: {off | prefix bytes << 20 | header byte << 24, |d0|}
-    uint32_t wave[3][WAVES][4];                 // {wave exclusive offset in the tile, wave bytes, 32-plane groups, -}
+    uint64_t wtot[WAVES];                      // per-wave byte counts
+    uint64_t excl;                             // the tile's exclusive prefix (wave 0's lookback)
+    uint32_t over[WAVES];                      // the wave's payload did not fit its staging area
+    Meta meta[WAVES][32];                      // per-block metadata of the chunk in work
+    uint32_t aux[WAVES][32];                   // sign words of the chunk's 32-plane 2D/3D groups
+    uint4 stage[WAVES][kStageBytes / 16 + 1];  // the wave's payload, as it will appear in the stream (+ a spare word)
 };
 
-// Processes tiles tile, tile + gridDim.x, ... while they belong to items of this
-// variant; returns with `tile` at the first tile it did not process and `it` at
-// that tile's item (or tile >= ntiles).
-//
-// Two-deep software pipeline built around gfx9's in-order vmcnt (stores and
-// loads share one counter, so a wait for any load also waits for every older
-// store): iteration i
-//   1. resolves the lookback of t_{i-2} (its first window was loaded at the
-//      start of iteration i-1) and writes t_{i-2}'s header bytes and stream
-//      words from LDS (stores are inline asm the compiler's waits do not see);
-//   2. loads the first lookback window of t_{i-1} (whose byte count and those
-//      of its round were published at the end of iteration i-1);
-//   3. runs pass 1 of t_i on data loaded during iteration i-1 and, group by
-//      group, loads t_{i+1} into the same registers;
-//   4. publishes t_i's byte count and parks its words and metadata in LDS.
-// Every wait of iteration i is for loads issued about one pass 1 earlier, so
-// no memory latency is exposed while the lookback has a whole iteration to
-// see its predecessors. The last tiles of an item (per workgroup) drain the
-// pipeline in extra iterations without a pass 1.
-template <int DIM, int MODE, class T, int G, int WAVES, int GMAX, class Src>
-__device__ __forceinline__ void compress_run(const Src& src, uint32_t& it, uint64_t& tile, const uint64_t ntiles,
-                                             const ScanCtx& sc, SmemC<WAVES, GMAX>& sm) {
+// Shared memory of a decompress workgroup.
+template <int WAVES, int GMAX>
+struct SmemD {
+    uint64_t wtot[WAVES];             // per-wave byte counts
+    uint64_t excl;                    // the tile's exclusive prefix (wave 0's lookback)
+    Meta meta[WAVES][2 * GMAX];       // per-block metadata
+    uint32_t aux[WAVES][2 * GMAX];    // outlier |d| of element 0 per block
+};
+
+// The wave-uniform part of the loads of a wave's G groups.
+template <int DIM, class T, int G>
+struct GroupLoads {
+    // Loads x[g] for the wave's groups grp0 .. grp0+G-1. FULL: every element
+    // lies inside the array (no per-element checks); otherwise out-of-range
+    // lanes read 0 and vmask records the valid (group, lane) pairs per group.
+    template <bool FULL>
+    __device__ __forceinline__ static void run(const T* __restrict__ in, const Geom& geo, const LaneGeo<DIM>& lg,
+                                               uint64_t grp0, int lane, T (&x)[G], uint64_t& vmask) {
+        if constexpr (FULL) {
+            vmask = (G == 64) ? ~0ull : (1ull << G) - 1;
+            if constexpr (DIM == 1) {
+                const T* p = in + grp0 * 64 + (uint64_t)lane;
+#pragma unroll
+                for (int g = 0; g < G; ++g) x[g] = __builtin_nontemporal_load(p + 64 * g);
+            } else {
+                Walker wk;
+                wk.init(grp0, geo);
+                if (wk.bx + G <= geo.Bx) {
+                    // the G blocks are consecutive along x: origins 8 (2D) or 4 (3D) elements apart
+                    const T* p = in + block_origin<DIM>(geo, wk) + lg.off;
+#pragma unroll
+                    for (int g = 0; g < G; ++g) x[g] = __builtin_nontemporal_load(p + (DIM == 2 ? 8 : 4) * g);
+                } else {
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        x[g] = __builtin_nontemporal_load(in + block_origin<DIM>(geo, wk) + lg.off);
+                        wk.next(geo);
+                    }
+                }
+            }
+        } else {
+            vmask = 0;
+            Walker wk;
+            if constexpr (DIM != 1) wk.init(grp0, geo);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                uint64_t idx;
+                const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
+                x[g] = valid ? __builtin_nontemporal_load(in + idx) : T(0);
+                vmask |= (uint64_t)valid << g;
+                if constexpr (DIM != 1) wk.next(geo);
+            }
+        }
+    }
+};
+
+// Whether every element of groups grp0 .. grp0+G-1 lies inside the array.
+template <int DIM, int G>
+__device__ __forceinline__ bool groups_full(const Geom& geo, uint64_t grp0) {
+    if constexpr (DIM == 1) return (grp0 + G) * 64 <= geo.n;
+    else return geo.aligned && grp0 + G <= geo.nblocks;
+}
+
+// Compresses tile `tile` of array `cur` (every wave of the workgroup calls it).
+// Each wave runs its kChunks chunks of G groups in turn:
+//   1. loads (one element per lane per group; a chunk's G loads are issued
+//      before the previous chunk is processed);
+//   2. pass 1, per group: quantise, Lorenzo residual, mirror, sign|magnitude,
+//      bit transpose; nonzero-plane masks go to the block's bookkeeping lane;
+//   3. bookkeeping, lane j <-> block j of the chunk: header byte (stored at
+//      once, its position is fixed), size, offset; metadata through LDS;
+//   4. the chunk's stream words are written into the wave's LDS staging area
+//      exactly as they will appear in the stream.
+// The wave totals then combine in LDS, wave 0 runs the lookback (publishing
+// the tile's byte count first, then its inclusive prefix), and every wave
+// copies its staged payload to the stream with 16-byte stores. A wave whose
+// payload did not fit its staging area runs its chunks again and stores the
+// words directly.
+template <int DIM, int MODE, class T, int WAVES>
+__device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t tile, const ScanCtx& sc,
+                                              SmemC<WAVES>& sm) {
+    constexpr int G = ShapeC<T>::G;              // groups per chunk
     constexpr int S = (DIM == 1) ? 4 : 8;        // bytes per plane
     constexpr int BPG = (DIM == 1) ? 2 : 1;      // blocks per group
-    constexpr int NB = BPG * G;                  // blocks per wave
-    constexpr int KLB = (16 / WAVES > 1) ? 16 / WAVES : 1;   // lookback words per lane
+    constexpr int NB = BPG * G;                  // blocks per chunk
+    static_assert(NB <= 32 && 2 * G <= 32, "shared metadata sized for 32 blocks per chunk");
+
+    const int lane = (int)__lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
+    const Geom& geo = cur.geo;
+    const T r = (T)cur.param;
+    const uint64_t lt = tile - cur.tile0;
+    uint8_t* __restrict__ const dst = (uint8_t*)cur.dst;
+    const uint64_t wgrp0 = (lt * WAVES + (uint64_t)wave) * (kChunks * G);   // this wave's first group
+
+    LaneGeo<DIM> lg;
+    lg.init(lane, geo);
+    const int pl = pred_lane<DIM>(lane);
+    const int pl_src = pl < 0 ? lane : pl;
+    XposeConsts xc;
+    xc.init(lane);
+    // after the transpose, element 0 of a block is bit 7 of its plane words: in
+    // both halves for 1D (two blocks), only in the low half for 2D/3D
+    const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
+    const uint32_t wo = word_offset<DIM>(l, h, true);        // this lane's word in a block's payload
+    const uint32_t wo_wide = word_offset<DIM>(l, h, false);  // same, 32-plane 2D/3D layout
+
+    uint64_t excl = 0, wexcl = 0, ttot = 0;
+    uint32_t wave_tot = 0;
+    bool over = false;
+    for (int pass = 0;; ++pass) {
+        // pass 0: header bytes and staging; pass 1 (a wave that did not fit):
+        // direct stores at gbase
+        const bool direct = pass == 1;
+        uint8_t* const gbase = dst + uniform64(geo.H + excl + wexcl);
+        wave_tot = 0;
+        // Processes chunk c from xv (vmask: valid elements per group); roll(g)
+        // runs right after group g's value is consumed (the fast path loads the
+        // next chunk's group g into xv[g] there).
+        auto process = [&](int c, T (&xv)[G], const uint64_t vmask, auto&& roll) {
+            const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
+
+            // ---- pass 1 ---------------------------------------------------------
+            uint32_t W[G];
+            uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
+            uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
+            static_for<0, G>([&](auto gc) {
+                constexpr int g = decltype(gc)::value;
+                const uint32_t q = quantize<T>(xv[g], r);   // x = 0 outside the array gives q = 0
+                roll(g);
+                uint32_t d;
+                if constexpr (MODE == kFixed) {
+                    d = q;
+                } else if constexpr (DIM == 1) {
+                    uint32_t qp = dpp<kDppWaveShr1>(q);
+                    qp = (lane & 31) ? qp : 0u;
+                    d = q - qp;
+                } else {
+                    uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
+                    qp = (pl >= 0) ? qp : 0u;
+                    d = q - qp;
+                }
+                if constexpr (MODE != kFixed) d = ((vmask >> g) & 1) ? d : 0u;   // residuals of padding are 0
+                const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
+                const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
+                uint64_t m1, m2 = 0;
+                bool wide = false;
+                if constexpr (DIM != 1) wide = ballot((int32_t)a < 0) != 0;
+                if (!wide) {
+                    // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
+                    // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
+                    W[g] = xpose32(bfi(0x80000000u, dm, a), xc);
+                    m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
+                    if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
+                } else {
+                    W[g] = xpose32(a, xc);
+                    m1 = ballot(W[g] != 0);
+                    if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0);
+                    const uint64_t sg = ballot((int32_t)dm < 0);
+                    if (lane == 0) {
+                        sm.aux[wave][2 * g] = (uint32_t)sg;
+                        sm.aux[wave][2 * g + 1] = (uint32_t)(sg >> 32);
+                    }
+                    big |= 1u << g;
+                }
+                if constexpr (DIM == 1) {
+                    bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
+                    bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
+                    if constexpr (MODE == kOutlier) {
+                        bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
+                        bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
+                        bko = wrlane<2 * g>(rdlane(a, 7), bko);
+                        bko = wrlane<2 * g + 1>(rdlane(a, 39), bko);
+                    }
+                } else {
+                    bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
+                    if constexpr (MODE == kOutlier) {
+                        bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
+                        bko = wrlane<g>(rdlane(a, 7), bko);
+                    }
+                }
+            });
+
+            // ---- bookkeeping: lane j <-> block j of this chunk ---------------------
+            const bool blk = lane < NB;
+            uint32_t byte;
+            if constexpr (MODE == kOutlier) byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
+            else byte = bitlen(bk1);
+            const BlockView v = view_of<MODE, S>(byte);
+            const uint32_t sz = blk ? v.size : 0u;
+            const uint32_t incl = scan_wave(sz);
+            const uint32_t ctot = rdlane(incl, kWaveSize - 1);
+            const uint32_t off = wave_tot + incl - sz;   // block offset within the wave's payload
+            if (!direct) {
+                // header bytes (lane j: block j); 2D/3D pad bytes [nblocks, H) are 0
+                const uint64_t b0 = (DIM == 1) ? 2 * grp0 : grp0;
+                if (blk && (DIM == 1 || b0 + (uint64_t)lane < geo.H)) dst[b0 + (uint64_t)lane] = (uint8_t)byte;
+            }
+            const bool put = direct || (!over && wave_tot + ctot <= kStageBytes);
+            over |= !put;
+            if (put) {
+                if (blk) sm.meta[wave][lane] = make_meta(off, v, sz != 0);
+                const Meta* const mrow = &sm.meta[wave][DIM == 1 ? h : 0];
+                if (!direct) {
+                    // staging: every lane writes every group (lanes without a word
+                    // of the block write into the spare word past the area), so
+                    // the metadata reads need no branches
+                    lds_u8* const sbase = (lds_u8*)sm.stage[wave];
+                    if constexpr (MODE == kOutlier) {
+                        // |d| of element 0 (4 bytes: the sign word written next
+                        // covers the bytes past the prefix)
+                        if (blk && v.sel) *(lds_u32u*)(sbase + off) = bko;
+                    }
+                    Meta mnext = mrow[0];
+                    static_for<0, G>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        const Meta m = mnext;
+                        if constexpr (g + 1 < G) mnext = mrow[BPG * (g + 1)];   // one group ahead
+                        const bool wideg = (DIM != 1) && ((big >> g) & 1);
+                        uint32_t w = W[g];
+                        if constexpr (MODE == kOutlier) {
+                            if (m.sel() && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
+                        }
+                        const uint32_t o = m.has(l) ? m.off() + (wideg ? wo_wide : wo) : kStageBytes;
+                        *(lds_u32u*)(sbase + o) = w;
+                        if constexpr (DIM != 1) {
+                            if (wideg && m.nz() && lane < 2) *(lds_u32u*)(sbase + m.off() + 4 * lane) = sm.aux[wave][2 * g + lane];
+                        }
+                    });
+                } else {
+                    if constexpr (MODE == kOutlier) {
+                        if (blk && v.sel) store_le(gbase + off, bko, v.nb);   // |d| of element 0, little-endian
+                    }
+                    static_for<0, G>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        const Meta m = mrow[BPG * g];
+                        const bool wideg = (DIM != 1) && ((big >> g) & 1);
+                        uint32_t w = W[g];
+                        if constexpr (MODE == kOutlier) {
+                            if (m.sel() && l != 31) w &= not_e0;
+                        }
+                        if (m.has(l)) store_u32u(gbase + (m.off() + (wideg ? wo_wide : wo)), w);
+                        if constexpr (DIM != 1) {
+                            if (wideg && m.nz() && lane < 2) store_u32u(gbase + (m.off() + 4 * lane), sm.aux[wave][2 * g + lane]);
+                        }
+                    });
+                }
+            }
+            wave_tot += ctot;
+        };
+        const T* __restrict__ const in = (const T*)cur.src;
+        // first element of chunk c's first group for this lane; with a fast
+        // tile, group g of the chunk starts `step` * g elements later
+        constexpr uint64_t step = (DIM == 1) ? 64 : (DIM == 2 ? 8 : 4);
+        auto chunk_ptr = [&](int c) -> const T* {
+            const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
+            if constexpr (DIM == 1) {
+                return in + grp0 * 64 + (uint64_t)lane;
+            } else {
+                Walker wk;
+                wk.init(grp0, geo);
+                return in + block_origin<DIM>(geo, wk) + lg.off;
+            }
+        };
+        // fast tile: every element inside the array and (2D/3D) each chunk's
+        // blocks consecutive along x
+        const bool fast = groups_full<DIM, kChunks * G>(geo, wgrp0) && (DIM == 1 || geo.Bx % G == 0);
+        if (fast) {
+            T xv[G];
+            const T* p = chunk_ptr(0);
+#pragma unroll
+            for (int g = 0; g < G; ++g) xv[g] = __builtin_nontemporal_load(p + step * g);
+            for (int c = 0; c < kChunks; ++c) {
+                // the next chunk (the last chunk re-reads itself from cache)
+                const T* const pn = chunk_ptr(c + 1 < kChunks ? c + 1 : c);
+                process(c, xv, ~0ull, [&](int g) { xv[g] = __builtin_nontemporal_load(pn + step * g); });
+            }
+        } else {
+            for (int c = 0; c < kChunks; ++c) {
+                T xv[G];
+                uint64_t vmask;
+                GroupLoads<DIM, T, G>::template run<false>(in, geo, lg, wgrp0 + (uint64_t)c * G, lane, xv, vmask);
+                process(c, xv, vmask, [](int) {});
+            }
+        }
+        if (direct) return;
+
+        // ---- the tile's offset (wave 0) ---------------------------------------------
+        if (lane == 0) {
+            sm.wtot[wave] = wave_tot;
+            sm.over[wave] = over;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            const uint64_t t = sm.wtot[w];
+            wexcl += (w < wave) ? t : 0;
+            ttot += t;
+        }
+        if (wave == 0) {
+            const uint64_t e = wave_lookback<kLookbackWords>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
+                return compress_tile_bytes<DIM, MODE, T, kChunks * G, WAVES>((const T*)cur.src, geo, r,
+                                                                            p - cur.tile0, lane);
+            });
+            if (lane == 0) {
+                sm.excl = e;
+                if (lt == geo.ntiles - 1 && cur.total) *cur.total = geo.H + e + ttot;
+            }
+        }
+        __syncthreads();
+        excl = sm.excl;
+        if (geo.H + excl + ttot > cur.limit) {   // never write past the caller's buffer
+            if (wave == 0 && lane == 0) atomicOr(cur.err, kErrOverflow);
+            return;
+        }
+        if (over) continue;   // run the chunks again, storing directly
+
+        // ---- copy the staged payload: 16 bytes per lane, then the tail bytes ---------
+        uint8_t* const obase = dst + uniform64(geo.H + excl + wexcl);
+        typedef uint4 __attribute__((aligned(1))) u128u;
+        const uint32_t n16 = wave_tot & ~15u;
+        for (uint32_t i = 16u * (uint32_t)lane; i < n16; i += 16u * kWaveSize)
+            *(u128u*)(obase + i) = sm.stage[wave][i / 16];
+        if ((uint32_t)lane < (wave_tot & 15u)) obase[n16 + lane] = ((const uint8_t*)sm.stage[wave])[n16 + lane];
+        return;
+    }
+}
+
+// Decompresses tile `tile` of array `cur` (every wave of the workgroup calls it):
+//   1. bookkeeping lanes (one per block of the wave) read the header bytes and
+//      scan the block sizes; wave 0 runs the lookback on the tile total;
+//   2. each wave loads the stream words of its G groups (all in flight at
+//      once), transposes them back, rebuilds q along the Lorenzo chains and
+//      stores (T)q * eb * 2.
+template <int DIM, int MODE, class T, int G, int WAVES, int GMAX>
+__device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t tile, const ScanCtx& sc,
+                                                SmemD<WAVES, GMAX>& sm) {
+    constexpr int S = (DIM == 1) ? 4 : 8;
+    constexpr int BPG = (DIM == 1) ? 2 : 1;
+    constexpr int NB = BPG * G;
     static_assert(NB <= kWaveSize, "one bookkeeping lane per block");
     static_assert(G <= GMAX, "shared memory sized for the largest variant");
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
-    const int pl = pred_lane<DIM>(lane);
-    const int pl_src = pl < 0 ? lane : pl;
-    XposeConsts xc;
-    xc.init(lane);
-    const uint32_t wo = word_offset<DIM>(l, h, true);        // this lane's word in a block's payload
-    const uint32_t wo_wide = word_offset<DIM>(l, h, false);  // same, 32-plane 2D/3D layout
-    // after the transpose, element 0 of a block is bit 7 of its plane words: in
-    // both halves for 1D (two blocks), only in the low half for 2D/3D
-    const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
-    const uint32_t stride = gridDim.x;
-
-    Item cur = src.get(it);
-    LaneGeo<DIM> lg;
-    lg.init(lane, cur.geo);
-
-    T xv[G];
-    uint64_t vmask = 0;   // bit g: this lane's element of group g lies inside the array
-    auto load_tile = [&](uint64_t lt) {
-        const Geom& geo = cur.geo;
-        const T* __restrict__ in = (const T*)cur.src;
-        const uint64_t g0 = (lt * WAVES + (uint64_t)wave) * G;
-        Walker wk;
-        if constexpr (DIM != 1) wk.init(g0, geo);
-        vmask = 0;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            uint64_t idx;
-            const bool valid = elem_of<DIM>(geo, lg, wk, g0 + g, lane, idx);
-            xv[g] = valid ? __builtin_nontemporal_load(in + idx) : T(0);
-            vmask |= (uint64_t)valid << g;
-            if constexpr (DIM != 1) wk.next(geo);
-        }
-    };
-
-    // resolve the lookback of tile t (slot s, ttot bytes) and write its stream (every wave)
-    uint64_t lbw[KLB];
-    auto finish = [&](uint32_t s, uint64_t t, uint64_t ttot) {
-        const Geom& geo = cur.geo;
-        const T r = (T)cur.param;
-        const uint64_t excl = wg_lookback<WAVES, KLB>(sc, t, cur.tile0, ttot, wave, lane, sm.lb,
-            [&](uint64_t p) { return compress_tile_bytes<DIM, MODE, T, G, WAVES>((const T*)cur.src, geo, r, p - cur.tile0, lane); },
-            true, &lbw);
-        if (t - cur.tile0 == geo.ntiles - 1 && threadIdx.x == 0 && cur.total) *cur.total = geo.H + excl + ttot;
-        const uint32_t wexcl = sm.wave[s][wave][0], wbytes = sm.wave[s][wave][1], big = sm.wave[s][wave][2];
-        const uint32_t ls0 = lane < NB ? sm.lane[s][wave][lane][0] : 0u;   // bookkeeping lanes only
-        // header bytes (lane j: block j of the wave); 2D/3D pad bytes [nblocks, H) are 0
-        const uint64_t grp0 = ((uint64_t)(t - cur.tile0) * WAVES + (uint64_t)wave) * G;
-        const uint64_t b0 = (DIM == 1) ? 2 * grp0 : grp0;
-        if (lane < NB && (DIM == 1 || b0 + (uint64_t)lane < geo.H))
-            st_async_b8((uint8_t*)cur.dst + b0, (uint32_t)lane, ls0 >> 24);
-        const uint64_t wave_base = excl + wexcl;
-        uint8_t* const wbase = (uint8_t*)cur.dst + uniform64(geo.H + wave_base);
-        const bool fits = geo.H + wave_base + wbytes <= cur.limit;   // never write past the caller's buffer
-        if (!fits && lane == 0) atomicOr(cur.err, kErrOverflow);
-        if (fits) {
-            if constexpr (MODE == kOutlier) {
-                const uint32_t pref = (ls0 >> 20) & 7;
-                if (lane < NB && pref) st_async_le(wbase, ls0 & 0xfffffu, sm.lane[s][wave][lane][1], pref);
-            }
-            static_for<0, G>([&](auto gc) {
-                constexpr int g = decltype(gc)::value;
-                const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
-                const bool wideg = (DIM != 1) && ((big >> g) & 1);
-                uint32_t w = sm.words[s][wave][g][lane];
-                if constexpr (MODE == kOutlier) {
-                    if (m.sel() && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
-                }
-                if (m.has(l)) st_async_b32(wbase, m.off() + (wideg ? wo_wide : wo), w);
-                if constexpr (DIM != 1) {
-                    if (wideg && m.nz() && lane < 2)
-                        st_async_b32(wbase, m.off() + 4 * lane, sm.aux[s][wave][2 * g + lane]);
-                }
-            });
-        }
-    };
-
-    for (;;) {   // items of this variant
-    load_tile(tile - cur.tile0);
-    // pipeline: npend tiles (0..2) wait for their stores; slot is where the
-    // tile entering pass 1 parks; the pending ones are tile - stride (slot - 1)
-    // and tile - 2 * stride (slot - 2), with byte counts ttot1 and ttot2
-    uint32_t npend = 0, slot = 0;
-    uint64_t ttot1 = 0, ttot2 = 0;
-    for (;;) {
     const Geom& geo = cur.geo;
-    const uint32_t s2 = (slot + 1) % 3;   // t_{i-2}
-    const uint32_t s1 = (slot + 2) % 3;   // t_{i-1}
-    // 1. oldest pending tile: lookback + stores
-    if (npend == 2) finish(s2, tile - 2 * (uint64_t)stride, ttot2);
-    // 2. first lookback window of the younger pending tile
-    if (npend >= 1) lookback_issue<WAVES, KLB>(sc, tile - stride, cur.tile0, wave, lane, lbw);
-
-    // 3. pass 1 of t_i
+    const uint8_t* __restrict__ cmp = (const uint8_t*)cur.src;
     const uint64_t lt = tile - cur.tile0;
-    const T r = (T)cur.param;
     const uint64_t grp0 = (lt * WAVES + (uint64_t)wave) * G;
-    const uint64_t next = tile + stride;
-    const bool same = next < cur.tile0 + geo.ntiles;   // next tile in this item
-    const T* __restrict__ in = (const T*)cur.src;
-    const uint64_t gn0 = ((next - cur.tile0) * WAVES + (uint64_t)wave) * G;
-    // The next tile's groups load during pass 1 when all of this wave's groups
-    // of it lie inside the array (then no per-element checks are needed);
-    // otherwise the tile loads after pass 1 with checks. The loads are issued
-    // on every iteration (a fixed number per iteration lets the compiler wait
-    // for exactly the loads it needs); without a next tile to roll in, every
-    // lane re-reads element 0.
-    bool roll = same;
-    uint64_t nbase = 0;
-    if constexpr (DIM == 1) {
-        roll = roll && (gn0 + G) * 64 <= geo.n;
-        nbase = gn0 * 64;
-    } else {
-        roll = roll && geo.aligned && gn0 + G <= geo.nblocks;
-    }
-    Walker wkn;
-    if constexpr (DIM != 1) {
-        if (roll) wkn.init(gn0, geo);
-    }
-    const uint32_t lofs = roll ? (uint32_t)lane : 0u;
-
-    uint32_t W[G];
-    uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
-    uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
-    // Tiles entirely inside the array skip the per-element validity select
-    // (fixed mode never needs it: x = 0 gives q = 0). 1D instantiates pass 1
-    // twice for that; 2D/3D branch per group (two copies cost more registers).
-    const bool full = ballot(vmask != (G == 64 ? ~0ull : (1ull << G) - 1)) == 0;
-    auto pass1 = [&](auto fullc) {
-    constexpr bool FULL = decltype(fullc)::value;
-    static_for<0, G>([&](auto gc) {
-        constexpr int g = decltype(gc)::value;
-        const uint32_t q = quantize<T>(xv[g], r);   // x = 0 outside the array gives q = 0
-        if constexpr (DIM == 1) {
-            xv[g] = __builtin_nontemporal_load(in + (roll ? nbase + 64 * g : 0) + lofs);
-        } else {
-            xv[g] = __builtin_nontemporal_load(in + (roll ? block_origin<DIM>(geo, wkn) + lg.off : 0));
-            if (roll) wkn.next(geo);
-        }
-        uint32_t d;
-        if constexpr (MODE == kFixed) {
-            d = q;
-        } else if constexpr (DIM == 1) {
-            uint32_t qp = dpp<kDppWaveShr1>(q);
-            qp = (lane & 31) ? qp : 0u;
-            d = q - qp;
-        } else {
-            uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
-            qp = (pl >= 0) ? qp : 0u;
-            d = q - qp;
-        }
-        if constexpr (MODE != kFixed && !FULL) {
-            if (DIM == 1 || !full) d = ((vmask >> g) & 1) ? d : 0u;
-        }
-        const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
-        const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
-        uint64_t m1, m2 = 0;
-        bool wide = false;
-        if constexpr (DIM != 1) wide = ballot((int32_t)a < 0) != 0;
-        if (!wide) {
-            // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
-            // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
-            const uint32_t w = xpose32(bfi(0x80000000u, dm, a), xc);
-            W[g] = w;
-            m1 = ballot(w != 0) & 0x7fffffff7fffffffull;
-            if constexpr (MODE == kOutlier) m2 = ballot((w & not_e0) != 0) & 0x7fffffff7fffffffull;
-        } else {
-            const uint32_t w = xpose32(a, xc);
-            W[g] = w;
-            m1 = ballot(w != 0);
-            if constexpr (MODE == kOutlier) m2 = ballot((w & not_e0) != 0);
-            const uint64_t sg = ballot((int32_t)dm < 0);
-            if (lane == 0) {
-                sm.aux[slot][wave][2 * g] = (uint32_t)sg;
-                sm.aux[slot][wave][2 * g + 1] = (uint32_t)(sg >> 32);
-            }
-            big |= 1u << g;
-        }
-        if constexpr (DIM == 1) {
-            bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
-            bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
-            if constexpr (MODE == kOutlier) {
-                bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
-                bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
-                bko = wrlane<2 * g>(rdlane(a, 7), bko);
-                bko = wrlane<2 * g + 1>(rdlane(a, 39), bko);
-            }
-        } else {
-            bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
-            if constexpr (MODE == kOutlier) {
-                bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
-                bko = wrlane<g>(rdlane(a, 7), bko);
-            }
-        }
-    });
-    };
-    if (DIM == 1 && full) pass1(std::true_type{});
-    else pass1(std::false_type{});
-    if (roll) vmask = (G == 64) ? ~0ull : (1ull << G) - 1;
-    else if (same) load_tile(next - cur.tile0);   // a partial next tile: loads with checks
 
     // ---- bookkeeping: lane j <-> block j of this wave -----------------------
-    const bool blk = lane < NB;
-    uint32_t byte;
-    if constexpr (MODE == kOutlier) byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
-    else byte = bitlen(bk1);
+    const uint64_t b = ((DIM == 1) ? 2 * grp0 : grp0) + (uint64_t)lane;
+    const bool blk = (lane < NB) && (DIM == 1 || b < geo.nblocks);
+    const uint32_t byte = blk ? (uint32_t)cmp[b] : 0u;
     const BlockView v = view_of<MODE, S>(byte);
     const uint32_t sz = blk ? v.size : 0u;
     const uint32_t incl = scan_wave(sz);
     const uint32_t off = incl - sz;
     const uint64_t wtot = rdlane(incl, kWaveSize - 1);
-    if (blk) {
-        sm.meta[slot][wave][lane] = make_meta(off, v, sz != 0);
-        sm.lane[slot][wave][lane][0] = off | (((blk && v.sel) ? v.nb : 0u) << 20) | (byte << 24);
-        sm.lane[slot][wave][lane][1] = bko;
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g) sm.words[slot][wave][g][lane] = W[g];
-    // combine the wave totals and publish the tile's byte count right away
-    if (lane == 0) sm.wtot[slot][wave] = wtot;
+    // every entry is rewritten: blocks past the end get an empty lane mask
+    if (lane < NB) sm.meta[wave][lane] = make_meta(off, v, blk && sz != 0);
+    if (lane == 0) sm.wtot[wave] = wtot;
     __syncthreads();
     uint64_t wexcl = 0, ttot = 0;
 #pragma unroll
     for (int w = 0; w < WAVES; ++w) {
-        const uint64_t t = sm.wtot[slot][w];
+        const uint64_t t = sm.wtot[w];
         wexcl += (w < wave) ? t : 0;
         ttot += t;
     }
-    if (wave == 0 && lane == 0)
-        st_status_async(sc.status, (uint32_t)tile, sc.tag | (tile == cur.tile0 ? kFlagPrefix : kFlagAgg) | ttot);
-    if (lane == 0) {   // read back by this wave only
-        sm.wave[slot][wave][0] = (uint32_t)wexcl;
-        sm.wave[slot][wave][1] = (uint32_t)wtot;
-        sm.wave[slot][wave][2] = big;
+    if (wave == 0) {
+        const uint64_t excl = wave_lookback<kLookbackWords>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
+            return header_tile_bytes<DIM, MODE, G, WAVES>(cmp, geo, p - cur.tile0, lane);
+        });
+        if (lane == 0) sm.excl = excl;
     }
-    npend = npend == 2 ? 2 : npend + 1;
-    slot = (slot + 1) % 3;
-    ttot2 = ttot1;
-    ttot1 = ttot;
+    __syncthreads();
 
-    tile = next;
-    if (!same) break;
-    }   // tiles of this item
-    // drain: the last one or two tiles of this item for this workgroup
-    {
-        const uint64_t last = tile - stride;
-        if (npend == 2) finish((slot + 1) % 3, last - stride, ttot2);
-        lookback_issue<WAVES, KLB>(sc, last, cur.tile0, wave, lane, lbw);
-        finish((slot + 2) % 3, last, ttot1);
+    const uint64_t wave_base = sm.excl + wexcl;
+    const uint8_t* const wbase = cmp + uniform64(geo.H + wave_base);
+    const bool intact = geo.H + wave_base + wtot <= cur.limit;
+    if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
+    if constexpr (MODE == kOutlier) {
+        // a selected block starts with nb <= 4 prefix bytes and then a 4-byte sign
+        // word, so one u32 read at the block start stays inside the block
+        uint32_t o = 0;
+        if (blk && v.sel && intact) {
+            o = load_u32u(wbase + off);
+            if (v.nb < 4) o &= (1u << (8 * v.nb)) - 1;
+        }
+        if (lane < NB) sm.aux[wave][lane] = o;
+        __builtin_amdgcn_wave_barrier();
     }
-    if (tile >= ntiles) return;
-    it = src.locate(tile, it);
-    if (src.variant(it) != cur.variant) return;
-    cur = src.get(it);
-    lg.init(lane, cur.geo);
-    }   // items
-}
-
-
-// ---------------------------------------------------------------------------
-// decompress: persistent loop, same contract as compress_run
-// ---------------------------------------------------------------------------
-// Shared memory of a decompress workgroup, sized for the largest variant.
-template <int WAVES, int GMAX>
-struct SmemD {
-    LookbackShared<WAVES> lb;
-    uint64_t wtot[2][WAVES];
-    Meta meta[2][WAVES][2 * GMAX];    // per-block metadata of the current and the pending tile
-    uint32_t aux[WAVES][2 * GMAX];    // outlier |d| of element 0 per block of the tile being decoded
-};
-
-// Same one-deep pipeline as compress_run: iteration i turns tile t_i's header
-// bytes into block sizes and publishes its byte count at once (its header
-// bytes were loaded during iteration i-1), then resolves the lookback of
-// t_{i-1} and decodes it.
-template <int DIM, int MODE, class T, int G, int WAVES, int GMAX, class Src>
-__device__ __forceinline__ void decompress_run(const Src& src, uint32_t& it, uint64_t& tile, const uint64_t ntiles,
-                                               const ScanCtx& sc, SmemD<WAVES, GMAX>& sm) {
-    constexpr int S = (DIM == 1) ? 4 : 8;
-    constexpr int BPG = (DIM == 1) ? 2 : 1;
-    constexpr int NB = BPG * G;
-    constexpr int KLB = (16 / WAVES > 1) ? 16 / WAVES : 1;   // lookback words per lane
-    static_assert(G <= GMAX, "shared memory sized for the largest variant");
-
-    const int lane = (int)__lane_id();
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
+    // ---- the stream words of all G groups -----------------------------------
+    const uint32_t wo = word_offset<DIM>(l, h, true);
+    const uint32_t wo_wide = word_offset<DIM>(l, h, false);
+    uint32_t W[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
+        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
+        W[g] = (intact && m.has(l)) ? load_u32u(wbase + (m.off() + (wideg ? wo_wide : wo))) : 0u;
+    }
+    // ---- decode -----------------------------------------------------------------
     XposeConsts xc;
     xc.init(lane);
-    const uint32_t wo = word_offset<DIM>(l, h, true);        // this lane's word in a block's payload
-    const uint32_t wo_wide = word_offset<DIM>(l, h, false);  // same, 32-plane 2D/3D layout
-    const uint64_t stride = gridDim.x;
-    Item cur = src.get(it);
     LaneGeo<DIM> lg;
-    lg.init(lane, cur.geo);
-    // header byte of this lane's block in a tile (lane j <-> block j of the wave)
-    auto header_byte = [&](uint64_t lt) -> uint32_t {
-        const uint64_t g0 = (lt * WAVES + (uint64_t)wave) * G;
-        const uint64_t b = (DIM == 1) ? 2 * g0 + (uint64_t)lane : g0 + (uint64_t)lane;
-        const bool ok = (lane < NB) && (DIM == 1 || b < cur.geo.nblocks);
-        return ok ? (uint32_t)((const uint8_t*)cur.src)[b] : 0u;
-    };
-    uint32_t byte = header_byte(tile - cur.tile0);
-
-    // the pending tile: block sizes known and byte count published
-    bool pend = false;
-    uint32_t pbuf = 0;
-    uint64_t p_tile = 0, p_wexcl = 0, p_ttot = 0, p_wtot = 0;
-    uint32_t p_off = 0, p_pref = 0;   // bookkeeping lane: block offset, outlier prefix bytes (0: none)
-
-    // lookback, stream loads, decode and stores of the pending tile (every wave)
-    auto flush = [&]() {
-        const Geom& geo = cur.geo;
-        const uint8_t* __restrict__ cmp = (const uint8_t*)cur.src;
-        T* __restrict__ out = (T*)cur.dst;
-        const T eb = (T)cur.param;
-        const uint64_t grp0 = ((p_tile - cur.tile0) * WAVES + (uint64_t)wave) * G;
-        const uint64_t excl = wg_lookback<WAVES, KLB>(sc, p_tile, cur.tile0, p_ttot, wave, lane, sm.lb,
-            [&](uint64_t p) { return header_tile_bytes<DIM, MODE, G, WAVES>(cmp, geo, p - cur.tile0, lane); },
-            true);
-        const uint64_t wave_base = excl + p_wexcl;
-        const uint8_t* const wbase = cmp + uniform64(geo.H + wave_base);
-        const bool intact = geo.H + wave_base + p_wtot <= cur.limit;
-        if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
+    lg.init(lane, geo);
+    T* __restrict__ out = (T*)cur.dst;
+    const T eb = (T)cur.param;
+    const bool full = groups_full<DIM, G>(geo, grp0);
+    Walker wk;
+    if constexpr (DIM != 1) wk.init(grp0, geo);
+    bool run = true;   // the groups' origins are consecutive (1D: always; 2D/3D: a run along x)
+    T* orun;
+    if constexpr (DIM == 1) {
+        orun = out + grp0 * 64 + (uint64_t)lane;
+    } else {
+        run = full && wk.bx + G <= geo.Bx;
+        orun = out + (run ? block_origin<DIM>(geo, wk) + lg.off : 0);
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
+        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
+        const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
+        uint32_t a, neg;
+        if (!wideg) {
+            neg = x >> 31;
+            a = x & 0x7fffffffu;
+        } else {   // 32 planes: the sign words sit in their own two words
+            const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
+            neg = (sw >> l) & 1;
+            a = x;
+        }
         if constexpr (MODE == kOutlier) {
-            // a selected block starts with nb <= 4 prefix bytes and then a 4-byte sign
-            // word, so one u32 read at the block start stays inside the block
-            uint32_t o = 0;
-            if (p_pref && intact) {
-                o = load_u32u(wbase + p_off);
-                if (p_pref < 4) o &= (1u << (8 * p_pref)) - 1;
-            }
-            if (lane < NB) sm.aux[wave][lane] = o;
-            __builtin_amdgcn_wave_barrier();
+            if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
         }
-        // ---- the stream words of all G groups -----------------------------------
-        uint32_t W[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const Meta m = sm.meta[pbuf][wave][BPG * g + (DIM == 1 ? h : 0)];
-            const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-            W[g] = (intact && m.has(l)) ? load_u32u(wbase + (m.off() + (wideg ? wo_wide : wo))) : 0u;
-        }
-        // ---- decode ---------------------------------------------------------------
-        Walker wk;
-        if constexpr (DIM != 1) wk.init(grp0, geo);
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const Meta m = sm.meta[pbuf][wave][BPG * g + (DIM == 1 ? h : 0)];
-            const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-            const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
-            uint32_t a, neg;
-            if (!wideg) {
-                neg = x >> 31;
-                a = x & 0x7fffffffu;
-            } else {   // 32 planes: the sign words sit in their own two words
-                const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
-                neg = (sw >> l) & 1;
-                a = x;
-            }
-            if constexpr (MODE == kOutlier) {
-                if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
-            }
-            const uint32_t dm = neg ? 0u - a : a;
-            const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
-            uint32_t q;
-            if constexpr (MODE == kFixed) q = d;
-            else q = inverse_lorenzo<DIM>(d, lane);
-            const bool zero_blk = !intact || !m.nz();
-            const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
+        const uint32_t dm = neg ? 0u - a : a;
+        const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
+        uint32_t q;
+        if constexpr (MODE == kFixed) q = d;
+        else q = inverse_lorenzo<DIM>(d, lane);
+        const bool zero_blk = !intact || !m.nz();
+        const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
+        if (full && run) {
+            orun[(DIM == 1 ? 64 : DIM == 2 ? 8 : 4) * g] = val;
+        } else {
             uint64_t idx;
             const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
             if (valid) out[idx] = val;
-            if constexpr (DIM != 1) wk.next(geo);
         }
-        pend = false;
-    };
-
-    for (uint32_t cbuf = 0;; cbuf ^= 1) {
-    const Geom& geo = cur.geo;
-    const uint64_t lt = tile - cur.tile0;
-    const uint64_t grp0 = (lt * WAVES + (uint64_t)wave) * G;
-
-    // ---- bookkeeping: lane j <-> block j of this wave -----------------------
-    const uint64_t blk_idx = (DIM == 1) ? 2 * grp0 + (uint64_t)lane : grp0 + (uint64_t)lane;
-    const bool blk = (lane < NB) && (DIM == 1 || blk_idx < geo.nblocks);
-    const BlockView v = view_of<MODE, S>(byte);
-    const uint32_t sz = blk ? v.size : 0u;
-    const uint32_t incl = scan_wave(sz);
-    const uint32_t off = incl - sz;
-    const uint64_t wtot = rdlane(incl, kWaveSize - 1);
-    // every entry is rewritten: blocks past the end get an empty lane mask, so no
-    // load ever uses an offset left over from an earlier tile or array
-    if (lane < NB) sm.meta[cbuf][wave][lane] = make_meta(off, v, blk && sz != 0);
-    if (lane == 0) sm.wtot[cbuf][wave] = wtot;
-    __syncthreads();
-    uint64_t wexcl = 0, ttot = 0;
-#pragma unroll
-    for (int w = 0; w < WAVES; ++w) {
-        const uint64_t t = sm.wtot[cbuf][w];
-        wexcl += (w < wave) ? t : 0;
-        ttot += t;
+        if constexpr (DIM != 1) wk.next(geo);
     }
-    if (wave == 0 && lane == 0) publish_tile(sc, tile, cur.tile0, ttot);
-
-    // the next tile's header bytes load while the pending tile decodes
-    const uint64_t next = tile + stride;
-    const bool same = next < cur.tile0 + geo.ntiles;
-    const uint32_t c_pref = (blk && v.sel) ? v.nb : 0u;
-    if (same) byte = header_byte(next - cur.tile0);
-
-    if (pend) flush();   // the previous tile's lookback and decode
-
-    pend = true;
-    pbuf = cbuf;
-    p_tile = tile;
-    p_wexcl = wexcl;
-    p_ttot = ttot;
-    p_wtot = wtot;
-    p_off = off;
-    p_pref = c_pref;
-
-    tile = next;
-    if (same) continue;
-    flush();   // last tile of this item for this workgroup
-    if (tile >= ntiles) return;
-    it = src.locate(tile, it);
-    if (src.variant(it) != cur.variant) return;
-    cur = src.get(it);
-    lg.init(lane, cur.geo);
-    byte = header_byte(tile - cur.tile0);
-    }   // tiles
 }
 
+// ---------------------------------------------------------------------------
+// single-array kernels: one workgroup per tile, tile = base + blockIdx.x (tiles
+// are numbered in launch order; arrays of more than one grid's tiles take
+// several launches)
+// ---------------------------------------------------------------------------
+template <int DIM, int MODE, class T, int WAVES>
+__global__ void __launch_bounds__(WAVES * kWaveSize, kCompressWavesPerSimd)
+compress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
+    __shared__ SmemC<WAVES> sm;
+    compress_tile<DIM, MODE, T, WAVES>(item, base + blockIdx.x, sc, sm);
+}
 
-// ---------------------------------------------------------------------------
-// single-array kernels: one item, every tile of the same variant
-// ---------------------------------------------------------------------------
-template <int DIM, int MODE, class T, int G, int WAVES>
+template <int DIM, int MODE, class T, int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize)
-compress_kernel(const Item item, const ScanCtx sc) {
-    __shared__ SmemC<WAVES, G> sm;
-    const OneItem src{item};
-    uint32_t it = 0;
-    uint64_t tile = blockIdx.x;
-    if (tile < item.geo.ntiles) compress_run<DIM, MODE, T, G, WAVES>(src, it, tile, item.geo.ntiles, sc, sm);
+decompress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
+    __shared__ SmemD<WAVES, kDecompressG> sm;
+    decompress_tile<DIM, MODE, T, kDecompressG, WAVES>(item, base + blockIdx.x, sc, sm);
 }
 
-template <int DIM, int MODE, class T, int G, int WAVES>
-__global__ void __launch_bounds__(WAVES * kWaveSize)
-decompress_kernel(const Item item, const ScanCtx sc) {
-    __shared__ SmemD<WAVES, G> sm;
-    const OneItem src{item};
-    uint32_t it = 0;
-    uint64_t tile = blockIdx.x;
-    if (tile < item.geo.ntiles) decompress_run<DIM, MODE, T, G, WAVES>(src, it, tile, item.geo.ntiles, sc, sm);
-}
-
-}  // namespace cuszp
-
-namespace cuszp {
-
 // ---------------------------------------------------------------------------
-// batched kernels: the arrays of a table, of any variant, in one persistent
-// launch. Workgroup w takes tiles w, w + B, ... of the concatenated tile space;
-// a tile's array is found by binary search on tile0, and each array's tiles
-// form their own lookback segment, so every array's stream is exactly its
-// single-array stream. f64 arrays use half as many groups per wave (tiles of
-// the same byte size as f32 tiles).
+// batched kernels: the arrays of a table, of any variant, in one launch with
+// one workgroup per tile of the concatenated tile space. A tile's array is
+// found by binary search on tile0, and each array's tiles form their own
+// lookback segment, so every array's stream is exactly its single-array
+// stream.
 // ---------------------------------------------------------------------------
-constexpr int kBatchGF32 = 16;
-constexpr int kBatchGF64 = 8;
-
 #define CUSZP_BATCH_CASES(RUN)                                                        \
-    case 0: RUN(1, kFixed, float, kBatchGF32); break;                                 \
-    case 1: RUN(1, kPlain, float, kBatchGF32); break;                                 \
-    case 2: RUN(1, kOutlier, float, kBatchGF32); break;                               \
-    case 3: RUN(2, kFixed, float, kBatchGF32); break;                                 \
-    case 4: RUN(2, kPlain, float, kBatchGF32); break;                                 \
-    case 5: RUN(2, kOutlier, float, kBatchGF32); break;                               \
-    case 6: RUN(3, kFixed, float, kBatchGF32); break;                                 \
-    case 7: RUN(3, kPlain, float, kBatchGF32); break;                                 \
-    case 8: RUN(3, kOutlier, float, kBatchGF32); break;                               \
-    case 9: RUN(1, kFixed, double, kBatchGF64); break;                                \
-    case 10: RUN(1, kPlain, double, kBatchGF64); break;                               \
-    case 11: RUN(1, kOutlier, double, kBatchGF64); break;                             \
-    case 12: RUN(2, kFixed, double, kBatchGF64); break;                               \
-    case 13: RUN(2, kPlain, double, kBatchGF64); break;                               \
-    case 14: RUN(2, kOutlier, double, kBatchGF64); break;                             \
-    case 15: RUN(3, kFixed, double, kBatchGF64); break;                               \
-    case 16: RUN(3, kPlain, double, kBatchGF64); break;                               \
-    case 17: RUN(3, kOutlier, double, kBatchGF64); break;                             \
-    default: return;
+    case 0: RUN(1, kFixed, float); break;                                             \
+    case 1: RUN(1, kPlain, float); break;                                             \
+    case 2: RUN(1, kOutlier, float); break;                                           \
+    case 3: RUN(2, kFixed, float); break;                                             \
+    case 4: RUN(2, kPlain, float); break;                                             \
+    case 5: RUN(2, kOutlier, float); break;                                           \
+    case 6: RUN(3, kFixed, float); break;                                             \
+    case 7: RUN(3, kPlain, float); break;                                             \
+    case 8: RUN(3, kOutlier, float); break;                                           \
+    case 9: RUN(1, kFixed, double); break;                                            \
+    case 10: RUN(1, kPlain, double); break;                                           \
+    case 11: RUN(1, kOutlier, double); break;                                         \
+    case 12: RUN(2, kFixed, double); break;                                           \
+    case 13: RUN(2, kPlain, double); break;                                           \
+    case 14: RUN(2, kOutlier, double); break;                                         \
+    case 15: RUN(3, kFixed, double); break;                                           \
+    case 16: RUN(3, kPlain, double); break;                                           \
+    case 17: RUN(3, kOutlier, double); break;                                         \
+    default: break;
 
 template <int WAVES>
-__global__ void __launch_bounds__(WAVES * kWaveSize)
-compress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, uint64_t ntiles, ScanCtx sc,
-                      const unsigned long long* __restrict__ epoch) {
-    __shared__ SmemC<WAVES, kBatchGF32> sm;
+__global__ void __launch_bounds__(WAVES * kWaveSize, kCompressWavesPerSimd)
+compress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx sc,
+                      const unsigned long long* __restrict__ epoch, const uint64_t base) {
+    __shared__ SmemC<WAVES> sm;
     sc.tag = (epoch[0] & kEpochMask) << kEpochShift;
     const ItemTable src{items, nitems};
-    uint64_t tile = blockIdx.x;
-    if (tile >= ntiles) return;
-    uint32_t it = src.locate(tile, 0);
-#define CUSZP_RUN_C(D, M, T, G) compress_run<D, M, T, G, WAVES>(src, it, tile, ntiles, sc, sm)
-    while (tile < ntiles) {
-        switch (src.variant(it)) { CUSZP_BATCH_CASES(CUSZP_RUN_C) }
-    }
+    const uint64_t tile = base + blockIdx.x;
+    const Item cur = src.get(src.locate(tile, 0));
+#define CUSZP_RUN_C(D, M, T) compress_tile<D, M, T, WAVES>(cur, tile, sc, sm)
+    switch (cur.variant) { CUSZP_BATCH_CASES(CUSZP_RUN_C) }
 #undef CUSZP_RUN_C
 }
 
 template <int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize)
-decompress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, uint64_t ntiles, ScanCtx sc,
-                        const unsigned long long* __restrict__ epoch) {
-    __shared__ SmemD<WAVES, kBatchGF32> sm;
+decompress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx sc,
+                        const unsigned long long* __restrict__ epoch, const uint64_t base) {
+    __shared__ SmemD<WAVES, kDecompressG> sm;
     sc.tag = (epoch[0] & kEpochMask) << kEpochShift;
     const ItemTable src{items, nitems};
-    uint64_t tile = blockIdx.x;
-    if (tile >= ntiles) return;
-    uint32_t it = src.locate(tile, 0);
-#define CUSZP_RUN_D(D, M, T, G) decompress_run<D, M, T, G, WAVES>(src, it, tile, ntiles, sc, sm)
-    while (tile < ntiles) {
-        switch (src.variant(it)) { CUSZP_BATCH_CASES(CUSZP_RUN_D) }
-    }
+    const uint64_t tile = base + blockIdx.x;
+    const Item cur = src.get(src.locate(tile, 0));
+#define CUSZP_RUN_D(D, M, T) decompress_tile<D, M, T, kDecompressG, WAVES>(cur, tile, sc, sm)
+    switch (cur.variant) { CUSZP_BATCH_CASES(CUSZP_RUN_D) }
 #undef CUSZP_RUN_D
 }
 

@@ -5,12 +5,12 @@ namespace cuszp {
 
 template <int D, int M, int W>
 static hipError_t launch_c(const Item& item, const ScanCtx& sc, hipStream_t st) {
-    return launch_persistent(compress_kernel<D, M, float, groups_per_wave(W), W>, W * kWaveSize, item.geo.ntiles, st,
+    return launch_tiles(compress_kernel<D, M, float, W>, W * kWaveSize, item.geo.ntiles, st,
                              item, sc);
 }
 template <int D, int M, int W>
 static hipError_t launch_d(const Item& item, const ScanCtx& sc, hipStream_t st) {
-    return launch_persistent(decompress_kernel<D, M, float, groups_per_wave(W), W>, W * kWaveSize, item.geo.ntiles, st,
+    return launch_tiles(decompress_kernel<D, M, float, W>, W * kWaveSize, item.geo.ntiles, st,
                              item, sc);
 }
 

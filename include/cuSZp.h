@@ -199,7 +199,8 @@ int cuSZp_batch_plan_results(cuszp_batch_plan_t plan, size_t* cmpSizes, int* sta
 /* Device array of per-array stream lengths written by each execute (compress). */
 const unsigned long long* cuSZp_batch_plan_device_sizes(cuszp_batch_plan_t plan);
 /* Times the candidate launch shapes on `stream` (runs the batch several times)
- * and keeps the fastest; returns its workgroup size in *waves and grid in *grid. */
+ * and keeps the fastest; returns its workgroup size (waves of 64 lanes) in
+ * *waves and the workgroups of its (first) launch, one per tile, in *grid. */
 int cuSZp_batch_plan_tune(cuszp_batch_plan_t plan, hipStream_t stream, int* waves, unsigned* grid);
 void cuSZp_batch_plan_destroy(cuszp_batch_plan_t plan);
 
