@@ -125,8 +125,8 @@ TilePlan plan_tiles(int dim, int type, bool compress, const Geom& g) {
     const uint64_t groups = (dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;   // 64-element units
     int waves = forced;
     if (!waves) waves = 4;
-    const uint64_t per_tile = compress ? compress_tile_groups(type == CUSZP_TYPE_DOUBLE, waves)
-                                       : decompress_tile_groups(waves);
+    (void)compress;   // both directions use the same tile shape
+    const uint64_t per_tile = tile_groups(type == CUSZP_TYPE_DOUBLE, waves);
     return TilePlan{waves, (groups + per_tile - 1) / per_tile};
 }
 

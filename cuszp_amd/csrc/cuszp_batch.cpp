@@ -71,9 +71,7 @@ int build_table(cuszp_batch_plan_s& p, hipStream_t st) {
         Geom g;
         make_geom(s.dim, s.nbEle, s.dims, g);
         const uint64_t groups = (s.dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;
-        const uint64_t per_tile = p.dir == CUSZP_BATCH_COMPRESS
-                                      ? compress_tile_groups(s.type == CUSZP_TYPE_DOUBLE, p.waves)
-                                      : decompress_tile_groups(p.waves);
+        const uint64_t per_tile = tile_groups(s.type == CUSZP_TYPE_DOUBLE, p.waves);
         g.ntiles = (groups + per_tile - 1) / per_tile;
         const bool c = p.dir == CUSZP_BATCH_COMPRESS;
         Item it = make_item(s.d_in, s.d_out, s.capacity, c ? p.d_sizes + i : nullptr, p.d_err + i,

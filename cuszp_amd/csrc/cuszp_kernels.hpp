@@ -733,19 +733,16 @@ constexpr int kLookbackWords = 4;
 // Words in front of every status array (window reads before tile 0 land there).
 constexpr uint64_t kStatusGuard = 64 * kLookbackWords;
 
-// Tile shapes. A compress wave works through kChunks chunks of ShapeC<T>::G
-// groups (one element per lane per group); a compress tile is WAVES * kChunks
-// * G groups (16384 f32 or 8192 f64 elements at 4 waves). A decompress wave
-// holds G groups at once; a decompress tile is WAVES * G groups.
+// Tile shape. A wave works through kChunks chunks of ShapeC<T>::G groups (one
+// element per lane per group); a tile is WAVES * kChunks * G groups (16384 f32
+// or 8192 f64 elements at 4 waves), for compress and decompress alike.
 template <class T> struct ShapeC;
 template <> struct ShapeC<float> { static constexpr int G = 8; };
 template <> struct ShapeC<double> { static constexpr int G = 4; };
 constexpr int kChunks = 8;
-constexpr int kDecompressG = 16;
-__host__ __device__ constexpr uint64_t compress_tile_groups(bool f64, int waves) {
+__host__ __device__ constexpr uint64_t tile_groups(bool f64, int waves) {
     return (uint64_t)waves * kChunks * (f64 ? 4 : 8);
 }
-__host__ __device__ constexpr uint64_t decompress_tile_groups(int waves) { return (uint64_t)waves * kDecompressG; }
 
 // Compressed bytes a compress wave stages in LDS; a wave whose payload is
 // larger (compression ratio below about 2.7 on its share of the tile)
@@ -768,12 +765,13 @@ struct SmemC {
 };
 
 // Shared memory of a decompress workgroup.
-template <int WAVES, int GMAX>
+template <int WAVES>
 struct SmemD {
     uint64_t wtot[WAVES];             // per-wave byte counts
     uint64_t excl;                    // the tile's exclusive prefix (wave 0's lookback)
-    Meta meta[WAVES][2 * GMAX];       // per-block metadata
-    uint32_t aux[WAVES][2 * GMAX];    // outlier |d| of element 0 per block
+    Meta meta[2][WAVES][32];          // per-block metadata of the chunk in work and the next one
+    uint32_t aux[2][WAVES][32];       // outlier |d| of element 0 per block, same two chunks
+    uint8_t hdr[WAVES][128];          // header bytes of the wave's blocks
 };
 
 // The wave-uniform part of the loads of a wave's G groups.
@@ -1096,20 +1094,26 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
     }
 }
 
-// Decompresses tile `tile` of array `cur` (every wave of the workgroup calls it):
-//   1. bookkeeping lanes (one per block of the wave) read the header bytes and
-//      scan the block sizes; wave 0 runs the lookback on the tile total;
-//   2. each wave loads the stream words of its G groups (all in flight at
-//      once), transposes them back, rebuilds q along the Lorenzo chains and
+// Decompresses tile `tile` of array `cur` (every wave of the workgroup calls
+// it); the tile has the compress tile's shape (kChunks chunks of G groups per
+// wave):
+//   1. each wave reads the header bytes of its blocks (parked in LDS) and sums
+//      their sizes; the wave totals combine in LDS and wave 0 runs the lookback;
+//   2. each wave works through its chunks: bookkeeping lanes (one per block)
+//      turn header bytes into offsets and metadata, every lane loads its
+//      stream words (the next chunk's loads are issued while this chunk
+//      decodes), transposes them back, rebuilds q along the Lorenzo chains and
 //      stores (T)q * eb * 2.
-template <int DIM, int MODE, class T, int G, int WAVES, int GMAX>
+template <int DIM, int MODE, class T, int WAVES>
 __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t tile, const ScanCtx& sc,
-                                                SmemD<WAVES, GMAX>& sm) {
+                                                SmemD<WAVES>& sm) {
+    constexpr int G = ShapeC<T>::G;
     constexpr int S = (DIM == 1) ? 4 : 8;
     constexpr int BPG = (DIM == 1) ? 2 : 1;
-    constexpr int NB = BPG * G;
-    static_assert(NB <= kWaveSize, "one bookkeeping lane per block");
-    static_assert(G <= GMAX, "shared memory sized for the largest variant");
+    constexpr int NB = BPG * G;              // blocks per chunk
+    constexpr int NBW = kChunks * NB;        // blocks per wave
+    static_assert(NBW <= 2 * kWaveSize, "header bytes of a wave: at most two per lane");
+    static_assert(NB <= 32, "shared metadata sized for 32 blocks per chunk");
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1117,19 +1121,21 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     const Geom& geo = cur.geo;
     const uint8_t* __restrict__ cmp = (const uint8_t*)cur.src;
     const uint64_t lt = tile - cur.tile0;
-    const uint64_t grp0 = (lt * WAVES + (uint64_t)wave) * G;
+    const uint64_t wgrp0 = (lt * WAVES + (uint64_t)wave) * (kChunks * G);   // this wave's first group
+    const uint64_t wb0 = (DIM == 1) ? 2 * wgrp0 : wgrp0;                     // and first block
 
-    // ---- bookkeeping: lane j <-> block j of this wave -----------------------
-    const uint64_t b = ((DIM == 1) ? 2 * grp0 : grp0) + (uint64_t)lane;
-    const bool blk = (lane < NB) && (DIM == 1 || b < geo.nblocks);
-    const uint32_t byte = blk ? (uint32_t)cmp[b] : 0u;
-    const BlockView v = view_of<MODE, S>(byte);
-    const uint32_t sz = blk ? v.size : 0u;
-    const uint32_t incl = scan_wave(sz);
-    const uint32_t off = incl - sz;
-    const uint64_t wtot = rdlane(incl, kWaveSize - 1);
-    // every entry is rewritten: blocks past the end get an empty lane mask
-    if (lane < NB) sm.meta[wave][lane] = make_meta(off, v, blk && sz != 0);
+    // ---- header bytes of the wave's blocks -------------------------------------
+    uint32_t wsum = 0;
+#pragma unroll
+    for (int i = 0; i < (NBW + kWaveSize - 1) / kWaveSize; ++i) {
+        const int j = i * kWaveSize + lane;
+        const uint64_t b = wb0 + (uint64_t)j;
+        const bool ok = j < NBW && (DIM == 1 || b < geo.nblocks);
+        const uint32_t byte = ok ? (uint32_t)cmp[b] : 0u;
+        if (j < NBW) sm.hdr[wave][j] = (uint8_t)byte;
+        wsum += ok ? view_of<MODE, S>(byte).size : 0u;
+    }
+    const uint32_t wtot = rdlane(scan_wave(wsum), kWaveSize - 1);   // < 2^32; a cross-lane scan: all lanes
     if (lane == 0) sm.wtot[wave] = wtot;
     __syncthreads();
     uint64_t wexcl = 0, ttot = 0;
@@ -1140,88 +1146,116 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         ttot += t;
     }
     if (wave == 0) {
-        const uint64_t excl = wave_lookback<kLookbackWords>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
-            return header_tile_bytes<DIM, MODE, G, WAVES>(cmp, geo, p - cur.tile0, lane);
+        const uint64_t e = wave_lookback<kLookbackWords>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
+            return header_tile_bytes<DIM, MODE, kChunks * G, WAVES>(cmp, geo, p - cur.tile0, lane);
         });
-        if (lane == 0) sm.excl = excl;
+        if (lane == 0) sm.excl = e;
     }
     __syncthreads();
 
-    const uint64_t wave_base = sm.excl + wexcl;
-    const uint8_t* const wbase = cmp + uniform64(geo.H + wave_base);
-    const bool intact = geo.H + wave_base + wtot <= cur.limit;
+    const uint64_t wave_base = geo.H + sm.excl + wexcl;
+    const uint8_t* const wbase = cmp + uniform64(wave_base);
+    const bool intact = wave_base + sm.wtot[wave] <= cur.limit;
     if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
-    if constexpr (MODE == kOutlier) {
-        // a selected block starts with nb <= 4 prefix bytes and then a 4-byte sign
-        // word, so one u32 read at the block start stays inside the block
-        uint32_t o = 0;
-        if (blk && v.sel && intact) {
-            o = load_u32u(wbase + off);
-            if (v.nb < 4) o &= (1u << (8 * v.nb)) - 1;
-        }
-        if (lane < NB) sm.aux[wave][lane] = o;
-        __builtin_amdgcn_wave_barrier();
-    }
-    // ---- the stream words of all G groups -----------------------------------
-    const uint32_t wo = word_offset<DIM>(l, h, true);
-    const uint32_t wo_wide = word_offset<DIM>(l, h, false);
-    uint32_t W[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
-        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-        W[g] = (intact && m.has(l)) ? load_u32u(wbase + (m.off() + (wideg ? wo_wide : wo))) : 0u;
-    }
-    // ---- decode -----------------------------------------------------------------
+
     XposeConsts xc;
     xc.init(lane);
     LaneGeo<DIM> lg;
     lg.init(lane, geo);
     T* __restrict__ out = (T*)cur.dst;
     const T eb = (T)cur.param;
-    const bool full = groups_full<DIM, G>(geo, grp0);
-    Walker wk;
-    if constexpr (DIM != 1) wk.init(grp0, geo);
-    bool run = true;   // the groups' origins are consecutive (1D: always; 2D/3D: a run along x)
-    T* orun;
-    if constexpr (DIM == 1) {
-        orun = out + grp0 * 64 + (uint64_t)lane;
-    } else {
-        run = full && wk.bx + G <= geo.Bx;
-        orun = out + (run ? block_origin<DIM>(geo, wk) + lg.off : 0);
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        const Meta m = sm.meta[wave][BPG * g + (DIM == 1 ? h : 0)];
-        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-        const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
-        uint32_t a, neg;
-        if (!wideg) {
-            neg = x >> 31;
-            a = x & 0x7fffffffu;
-        } else {   // 32 planes: the sign words sit in their own two words
-            const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
-            neg = (sw >> l) & 1;
-            a = x;
-        }
+    const uint32_t wo = word_offset<DIM>(l, h, true);
+    const uint32_t wo_wide = word_offset<DIM>(l, h, false);
+    constexpr uint64_t step = (DIM == 1) ? 64 : (DIM == 2 ? 8 : 4);
+    // fast tile: every element inside the array and (2D/3D) each chunk's blocks
+    // consecutive along x
+    const bool fast = groups_full<DIM, kChunks * G>(geo, wgrp0) && (DIM == 1 || geo.Bx % G == 0);
+
+    // Bookkeeping of chunk c (payload offset `coff` within the wave): metadata
+    // into slot s, the outlier |d0| of selected blocks into aux[s]; returns the
+    // chunk's byte count.
+    auto bookkeep = [&](int c, uint32_t coff, int s) -> uint32_t {
+        const bool blk = lane < NB;
+        const uint32_t byte = blk ? (uint32_t)sm.hdr[wave][c * NB + lane] : 0u;
+        const uint64_t b = wb0 + (uint64_t)(c * NB + lane);
+        const bool ok = blk && (DIM == 1 || b < geo.nblocks);
+        const BlockView v = view_of<MODE, S>(byte);
+        const uint32_t sz = ok ? v.size : 0u;
+        const uint32_t incl = scan_wave(sz);
+        const uint32_t off = coff + incl - sz;
+        // every entry is rewritten: blocks past the end get an empty lane mask
+        if (blk) sm.meta[s][wave][lane] = make_meta(off, v, ok && sz != 0);
         if constexpr (MODE == kOutlier) {
-            if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
+            // a selected block starts with nb <= 4 prefix bytes and then a 4-byte
+            // sign word, so one u32 read at the block start stays inside the block
+            uint32_t o = 0;
+            if (ok && v.sel && intact) {
+                o = load_u32u(wbase + off);
+                if (v.nb < 4) o &= (1u << (8 * v.nb)) - 1;
+            }
+            if (blk) sm.aux[s][wave][lane] = o;
         }
-        const uint32_t dm = neg ? 0u - a : a;
-        const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
-        uint32_t q;
-        if constexpr (MODE == kFixed) q = d;
-        else q = inverse_lorenzo<DIM>(d, lane);
-        const bool zero_blk = !intact || !m.nz();
-        const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
-        if (full && run) {
-            orun[(DIM == 1 ? 64 : DIM == 2 ? 8 : 4) * g] = val;
+        return rdlane(incl, kWaveSize - 1);
+    };
+    // stream word of this lane in group g of the chunk in slot s (0 when absent)
+    auto word = [&](int s, int g) -> uint32_t {
+        const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
+        const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
+        return (intact && m.has(l)) ? load_u32u(wbase + (m.off() + (wideg ? wo_wide : wo))) : 0u;
+    };
+
+    uint32_t W[G];
+    uint32_t coff = bookkeep(0, 0, 0);
+#pragma unroll
+    for (int g = 0; g < G; ++g) W[g] = word(0, g);
+    for (int c = 0; c < kChunks; ++c) {
+        const int s = c & 1;
+        const bool more = c + 1 < kChunks;
+        if (more) coff += bookkeep(c + 1, coff, s ^ 1);
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
+        Walker wk;
+        T* orun = out;
+        if constexpr (DIM == 1) {
+            orun = out + grp0 * 64 + (uint64_t)lane;
         } else {
-            uint64_t idx;
-            const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
-            if (valid) out[idx] = val;
+            wk.init(grp0, geo);
+            if (fast) orun = out + block_origin<DIM>(geo, wk) + lg.off;
         }
-        if constexpr (DIM != 1) wk.next(geo);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
+            const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
+            const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
+            if (more) W[g] = word(s ^ 1, g);        // the next chunk's word, loading while this one decodes
+            uint32_t a, neg;
+            if (!wideg) {
+                neg = x >> 31;
+                a = x & 0x7fffffffu;
+            } else {   // 32 planes: the sign words sit in their own two words
+                const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
+                neg = (sw >> l) & 1;
+                a = x;
+            }
+            if constexpr (MODE == kOutlier) {
+                if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[s][wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
+            }
+            const uint32_t dm = neg ? 0u - a : a;
+            const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
+            uint32_t q;
+            if constexpr (MODE == kFixed) q = d;
+            else q = inverse_lorenzo<DIM>(d, lane);
+            const bool zero_blk = !intact || !m.nz();
+            const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
+            if (fast) {
+                orun[step * g] = val;
+            } else {
+                uint64_t idx;
+                const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
+                if (valid) out[idx] = val;
+            }
+            if constexpr (DIM != 1) wk.next(geo);
+        }
     }
 }
 
@@ -1240,8 +1274,8 @@ compress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
 template <int DIM, int MODE, class T, int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize)
 decompress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
-    __shared__ SmemD<WAVES, kDecompressG> sm;
-    decompress_tile<DIM, MODE, T, kDecompressG, WAVES>(item, base + blockIdx.x, sc, sm);
+    __shared__ SmemD<WAVES> sm;
+    decompress_tile<DIM, MODE, T, WAVES>(item, base + blockIdx.x, sc, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1290,12 +1324,12 @@ template <int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize)
 decompress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx sc,
                         const unsigned long long* __restrict__ epoch, const uint64_t base) {
-    __shared__ SmemD<WAVES, kDecompressG> sm;
+    __shared__ SmemD<WAVES> sm;
     sc.tag = (epoch[0] & kEpochMask) << kEpochShift;
     const ItemTable src{items, nitems};
     const uint64_t tile = base + blockIdx.x;
     const Item cur = src.get(src.locate(tile, 0));
-#define CUSZP_RUN_D(D, M, T) decompress_tile<D, M, T, kDecompressG, WAVES>(cur, tile, sc, sm)
+#define CUSZP_RUN_D(D, M, T) decompress_tile<D, M, T, WAVES>(cur, tile, sc, sm)
     switch (cur.variant) { CUSZP_BATCH_CASES(CUSZP_RUN_D) }
 #undef CUSZP_RUN_D
 }

@@ -10,7 +10,7 @@
 namespace cuszp {
 
 // Tile plan: waves per workgroup (4 or 8) and the number of tiles (tile
-// shapes: compress_tile_groups, decompress_tile_groups).
+// shape: tile_groups).
 // CUSZP_WAVES in the environment overrides the choice (tuning experiments).
 struct TilePlan {
     int waves;
