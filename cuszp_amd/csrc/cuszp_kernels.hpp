@@ -480,6 +480,15 @@ struct Walker {
             if (++by == g.By) { by = 0; ++bz; }
         }
     }
+    // advance by n <= Bx blocks (crosses at most one row end)
+    __device__ __forceinline__ void step(const Geom& g, uint32_t n) {
+        b += n;
+        bx += n;
+        if (bx >= g.Bx) {
+            bx -= g.Bx;
+            if (++by == g.By) { by = 0; ++bz; }
+        }
+    }
 };
 
 // Per-lane geometry of an element of a group (natural element order e = lane).
@@ -882,6 +891,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
         const bool direct = pass == 1;
         uint8_t* const gbase = dst + uniform64(geo.H + excl + wexcl);
         wave_tot = 0;
+        const T* __restrict__ const in = (const T*)cur.src;
         // Processes chunk c from xv (vmask: valid elements per group); roll(g)
         // runs right after group g's value is consumed (the fast path loads the
         // next chunk's group g into xv[g] there).
@@ -889,65 +899,86 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
 
             // ---- pass 1 ---------------------------------------------------------
+            // WIDE = false: the branch-free form, which assumes no 2D/3D block
+            // holds a magnitude of 2^31 and reports the groups where one does;
+            // the chunk is then rerun from memory with WIDE = true.
             uint32_t W[G];
-            uint32_t bk1 = 0, bk2 = 0, bko = 0;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
-            uint32_t big = 0;                     // bit g: group g's block has a magnitude 2^31 (2D/3D)
-            static_for<0, G>([&](auto gc) {
-                constexpr int g = decltype(gc)::value;
-                const uint32_t q = quantize<T>(xv[g], r);   // x = 0 outside the array gives q = 0
-                roll(g);
-                uint32_t d;
-                if constexpr (MODE == kFixed) {
-                    d = q;
-                } else if constexpr (DIM == 1) {
-                    uint32_t qp = dpp<kDppWaveShr1>(q);
-                    qp = (lane & 31) ? qp : 0u;
-                    d = q - qp;
-                } else {
-                    uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
-                    qp = (pl >= 0) ? qp : 0u;
-                    d = q - qp;
-                }
-                if constexpr (MODE != kFixed) d = ((vmask >> g) & 1) ? d : 0u;   // residuals of padding are 0
-                const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
-                const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
-                uint64_t m1, m2 = 0;
-                bool wide = false;
-                if constexpr (DIM != 1) wide = ballot((int32_t)a < 0) != 0;
-                if (!wide) {
-                    // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
-                    // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
-                    W[g] = xpose32(bfi(0x80000000u, dm, a), xc);
-                    m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
-                    if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
-                } else {
-                    W[g] = xpose32(a, xc);
-                    m1 = ballot(W[g] != 0);
-                    if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0);
-                    const uint64_t sg = ballot((int32_t)dm < 0);
-                    if (lane == 0) {
-                        sm.aux[wave][2 * g] = (uint32_t)sg;
-                        sm.aux[wave][2 * g + 1] = (uint32_t)(sg >> 32);
+            uint32_t bk1, bk2, bko;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
+            uint32_t big;             // bit g: group g's block has a magnitude 2^31 (2D/3D)
+            auto pass1 = [&](auto widec, auto&& xval, const uint64_t vm, auto&& roll_fn) -> uint64_t {
+                constexpr bool WIDE = decltype(widec)::value;
+                bk1 = bk2 = bko = big = 0;
+                uint64_t anywide = 0;
+                static_for<0, G>([&](auto gc) {
+                    constexpr int g = decltype(gc)::value;
+                    const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array gives q = 0
+                    roll_fn(g);
+                    uint32_t d;
+                    if constexpr (MODE == kFixed) {
+                        d = q;
+                    } else if constexpr (DIM == 1) {
+                        uint32_t qp = dpp<kDppWaveShr1>(q);
+                        qp = (lane & 31) ? qp : 0u;
+                        d = q - qp;
+                    } else {
+                        uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
+                        qp = (pl >= 0) ? qp : 0u;
+                        d = q - qp;
                     }
-                    big |= 1u << g;
-                }
-                if constexpr (DIM == 1) {
-                    bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
-                    bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
-                    if constexpr (MODE == kOutlier) {
-                        bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
-                        bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
-                        bko = wrlane<2 * g>(rdlane(a, 7), bko);
-                        bko = wrlane<2 * g + 1>(rdlane(a, 39), bko);
+                    if constexpr (MODE != kFixed) d = ((vm >> g) & 1) ? d : 0u;   // residuals of padding are 0
+                    const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
+                    const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
+                    uint64_t m1, m2 = 0;
+                    bool wide = false;
+                    if constexpr (DIM != 1) {
+                        if constexpr (WIDE) wide = ballot((int32_t)a < 0) != 0;
+                        else anywide |= ballot((int32_t)a < 0);
                     }
-                } else {
-                    bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
-                    if constexpr (MODE == kOutlier) {
-                        bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
-                        bko = wrlane<g>(rdlane(a, 7), bko);
+                    if (!wide) {
+                        // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
+                        // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
+                        W[g] = xpose32(bfi(0x80000000u, dm, a), xc);
+                        m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
+                        if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
+                    } else {
+                        W[g] = xpose32(a, xc);
+                        m1 = ballot(W[g] != 0);
+                        if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0);
+                        const uint64_t sg = ballot((int32_t)dm < 0);
+                        if (lane == 0) {
+                            sm.aux[wave][2 * g] = (uint32_t)sg;
+                            sm.aux[wave][2 * g + 1] = (uint32_t)(sg >> 32);
+                        }
+                        big |= 1u << g;
                     }
+                    if constexpr (DIM == 1) {
+                        bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
+                        bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
+                        if constexpr (MODE == kOutlier) {
+                            bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
+                            bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
+                            bko = wrlane<2 * g>(rdlane(a, 7), bko);
+                            bko = wrlane<2 * g + 1>(rdlane(a, 39), bko);
+                        }
+                    } else {
+                        bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
+                        if constexpr (MODE == kOutlier) {
+                            bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
+                            bko = wrlane<g>(rdlane(a, 7), bko);
+                        }
+                    }
+                });
+                return anywide;
+            };
+            const uint64_t anywide = pass1(std::false_type{}, [&](int g) -> T { return xv[g]; }, vmask, roll);
+            if constexpr (DIM != 1) {
+                if (anywide) {   // rare: rerun the chunk with 32-plane blocks
+                    T xr[G];
+                    uint64_t vr;
+                    GroupLoads<DIM, T, G>::template run<false>(in, geo, lg, grp0, lane, xr, vr);
+                    pass1(std::true_type{}, [&](int g) -> T { return xr[g]; }, vr, [](int) {});
                 }
-            });
+            }
 
             // ---- bookkeeping: lane j <-> block j of this chunk ---------------------
             const bool blk = lane < NB;
@@ -1016,31 +1047,32 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             }
             wave_tot += ctot;
         };
-        const T* __restrict__ const in = (const T*)cur.src;
         // first element of chunk c's first group for this lane; with a fast
         // tile, group g of the chunk starts `step` * g elements later
         constexpr uint64_t step = (DIM == 1) ? 64 : (DIM == 2 ? 8 : 4);
-        auto chunk_ptr = [&](int c) -> const T* {
-            const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
-            if constexpr (DIM == 1) {
-                return in + grp0 * 64 + (uint64_t)lane;
-            } else {
-                Walker wk;
-                wk.init(grp0, geo);
-                return in + block_origin<DIM>(geo, wk) + lg.off;
-            }
-        };
         // fast tile: every element inside the array and (2D/3D) each chunk's
         // blocks consecutive along x
         const bool fast = groups_full<DIM, kChunks * G>(geo, wgrp0) && (DIM == 1 || geo.Bx % G == 0);
         if (fast) {
+            // wkn: the chunk whose loads are issued next (2D/3D; a run of G blocks
+            // along x, so the walker advances by G)
+            Walker wkn;
+            if constexpr (DIM != 1) wkn.init(wgrp0, geo);
+            auto chunk_ptr = [&](int c) -> const T* {
+                if constexpr (DIM == 1) return in + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
+                else return in + block_origin<DIM>(geo, wkn) + lg.off;
+            };
             T xv[G];
             const T* p = chunk_ptr(0);
 #pragma unroll
             for (int g = 0; g < G; ++g) xv[g] = __builtin_nontemporal_load(p + step * g);
             for (int c = 0; c < kChunks; ++c) {
                 // the next chunk (the last chunk re-reads itself from cache)
-                const T* const pn = chunk_ptr(c + 1 < kChunks ? c + 1 : c);
+                const bool more = c + 1 < kChunks;
+                if constexpr (DIM != 1) {
+                    if (more) wkn.step(geo, G);
+                }
+                const T* const pn = chunk_ptr(more ? c + 1 : c);
                 process(c, xv, ~0ull, [&](int g) { xv[g] = __builtin_nontemporal_load(pn + step * g); });
             }
         } else {
@@ -1173,7 +1205,9 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
 
     // Bookkeeping of chunk c (payload offset `coff` within the wave): metadata
     // into slot s, the outlier |d0| of selected blocks into aux[s]; returns the
-    // chunk's byte count.
+    // chunk's byte count. Loads that are not needed read the stream's first
+    // word instead (always inside the caller's buffer: H >= 4), so the code
+    // has no branches around memory operations.
     auto bookkeep = [&](int c, uint32_t coff, int s) -> uint32_t {
         const bool blk = lane < NB;
         const uint32_t byte = blk ? (uint32_t)sm.hdr[wave][c * NB + lane] : 0u;
@@ -1188,73 +1222,112 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         if constexpr (MODE == kOutlier) {
             // a selected block starts with nb <= 4 prefix bytes and then a 4-byte
             // sign word, so one u32 read at the block start stays inside the block
-            uint32_t o = 0;
-            if (ok && v.sel && intact) {
-                o = load_u32u(wbase + off);
-                if (v.nb < 4) o &= (1u << (8 * v.nb)) - 1;
-            }
+            const bool pre = ok && v.sel && intact;
+            uint32_t o = load_u32u(pre ? wbase + off : cmp);
+            o = pre ? (v.nb < 4 ? o & ((1u << (8 * v.nb)) - 1) : o) : 0u;
             if (blk) sm.aux[s][wave][lane] = o;
         }
         return rdlane(incl, kWaveSize - 1);
     };
-    // stream word of this lane in group g of the chunk in slot s (0 when absent)
+    // stream word of this lane in group g of the chunk in slot s, as loaded: a
+    // lane without a word loads the stream's first word (no branch around the
+    // load, so the compiler's waits stay exact), which decode() zeroes at the use
     auto word = [&](int s, int g) -> uint32_t {
         const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
         const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-        return (intact && m.has(l)) ? load_u32u(wbase + (m.off() + (wideg ? wo_wide : wo))) : 0u;
+        const bool has = intact && m.has(l);
+        return load_u32u(has ? wbase + (m.off() + (wideg ? wo_wide : wo)) : cmp);
     };
-
-    uint32_t W[G];
-    uint32_t coff = bookkeep(0, 0, 0);
-#pragma unroll
-    for (int g = 0; g < G; ++g) W[g] = word(0, g);
-    for (int c = 0; c < kChunks; ++c) {
-        const int s = c & 1;
-        const bool more = c + 1 < kChunks;
-        if (more) coff += bookkeep(c + 1, coff, s ^ 1);
-        __builtin_amdgcn_wave_barrier();
-        const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
-        Walker wk;
-        T* orun = out;
-        if constexpr (DIM == 1) {
-            orun = out + grp0 * 64 + (uint64_t)lane;
-        } else {
-            wk.init(grp0, geo);
-            if (fast) orun = out + block_origin<DIM>(geo, wk) + lg.off;
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
-            const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-            const uint32_t x = xpose32(W[g], xc);   // lane l: sign in bit 31 over the magnitude bits
-            if (more) W[g] = word(s ^ 1, g);        // the next chunk's word, loading while this one decodes
-            uint32_t a, neg;
-            if (!wideg) {
-                neg = x >> 31;
-                a = x & 0x7fffffffu;
-            } else {   // 32 planes: the sign words sit in their own two words
+    // decodes group g of the chunk in slot s from its loaded word w
+    auto decode = [&](int s, int g, uint32_t w, auto widec) -> T {
+        constexpr bool WIDE = decltype(widec)::value;   // 2D/3D 32-plane blocks possible
+        const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
+        const uint32_t x = xpose32((intact && m.has(l)) ? w : 0u, xc);
+        uint32_t a = x & 0x7fffffffu, neg = x >> 31;   // lane l: sign in bit 31 over the magnitude bits
+        if constexpr (WIDE) {
+            if (m.wide()) {   // 32 planes: the sign words sit in their own two words
                 const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
                 neg = (sw >> l) & 1;
                 a = x;
             }
-            if constexpr (MODE == kOutlier) {
-                if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[s][wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
-            }
-            const uint32_t dm = neg ? 0u - a : a;
-            const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
-            uint32_t q;
-            if constexpr (MODE == kFixed) q = d;
-            else q = inverse_lorenzo<DIM>(d, lane);
-            const bool zero_blk = !intact || !m.nz();
-            const T val = zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
-            if (fast) {
-                orun[step * g] = val;
-            } else {
+        }
+        if constexpr (MODE == kOutlier) {
+            if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[s][wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
+        }
+        const uint32_t dm = neg ? 0u - a : a;
+        const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
+        uint32_t q;
+        if constexpr (MODE == kFixed) q = d;
+        else q = inverse_lorenzo<DIM>(d, lane);
+        const bool zero_blk = !intact || !m.nz();
+        return zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
+    };
+
+    // 2D/3D blocks of 32 planes (not in outlier streams) take the general loop
+    bool wide_blocks = false;
+    if constexpr (DIM != 1 && MODE != kOutlier) {
+        uint32_t r32 = 0;
+#pragma unroll
+        for (int i = 0; i < (NBW + kWaveSize - 1) / kWaveSize; ++i) {
+            const int j = i * kWaveSize + lane;
+            if (j < NBW) r32 |= sm.hdr[wave][j] >= 32;
+        }
+        wide_blocks = ballot(r32 != 0) != 0;
+    }
+    if (fast && !wide_blocks) {
+        // ---- every element inside the array, chunks along x: no branches ----------
+        // Chunks go in pairs through two word buffers: a chunk's loads are
+        // issued before the stores of the chunk decoded meanwhile (loads and
+        // stores retire in order on this GPU, so a load issued after a store
+        // would also wait for that store).
+        Walker wkc;   // 2D/3D: the chunk being decoded
+        if constexpr (DIM != 1) wkc.init(wgrp0, geo);
+        auto decode_chunk = [&](int c, int s, const uint32_t (&W)[G]) {
+            T* orun;
+            if constexpr (DIM == 1) orun = out + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
+            else orun = out + block_origin<DIM>(geo, wkc) + lg.off;
+#pragma unroll
+            for (int g = 0; g < G; ++g) orun[step * g] = decode(s, g, W[g], std::false_type{});
+            if constexpr (DIM != 1) wkc.step(geo, G);
+        };
+        static_assert(kChunks % 2 == 0, "chunks go in pairs");
+        uint32_t WA[G], WB[G];
+        uint32_t coff = bookkeep(0, 0, 0);
+#pragma unroll
+        for (int g = 0; g < G; ++g) WA[g] = word(0, g);
+        for (int c = 0; c < kChunks; c += 2) {
+            coff += bookkeep(c + 1, coff, 1);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int g = 0; g < G; ++g) WB[g] = word(1, g);
+            decode_chunk(c, 0, WA);
+            // the chunk after next (past the last one: chunk c + 1 again, unused)
+            coff += bookkeep(c + 2 < kChunks ? c + 2 : c + 1, coff, 0);
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int g = 0; g < G; ++g) WA[g] = word(0, g);
+            decode_chunk(c + 1, 1, WB);
+        }
+    } else {
+        // ---- general: per-element checks, 32-plane blocks -------------------------
+        Walker wk;
+        if constexpr (DIM != 1) wk.init(wgrp0, geo);
+        uint32_t coff = 0;
+        for (int c = 0; c < kChunks; ++c) {
+            coff += bookkeep(c, coff, 0);
+            __builtin_amdgcn_wave_barrier();
+            uint32_t W[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) W[g] = word(0, g);
+            const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const T val = decode(0, g, W[g], std::true_type{});
                 uint64_t idx;
-                const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
-                if (valid) out[idx] = val;
+                if (elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx)) out[idx] = val;
+                if constexpr (DIM != 1) wk.next(geo);
             }
-            if constexpr (DIM != 1) wk.next(geo);
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
