@@ -783,6 +783,16 @@ struct SmemD {
     uint8_t hdr[WAVES][128];          // header bytes of the wave's blocks
 };
 
+// Load of one element per lane. 1D loads cover whole cache lines and stream
+// (nontemporal); a 2D/3D group reads 16 or 32 bytes of each of its rows' lines,
+// whose other bytes belong to the chunk's next groups, so those loads keep
+// the lines cached.
+template <int DIM, class T>
+__device__ __forceinline__ T load_elem(const T* p) {
+    if constexpr (DIM == 1) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 // The wave-uniform part of the loads of a wave's G groups.
 template <int DIM, class T, int G>
 struct GroupLoads {
@@ -797,7 +807,7 @@ struct GroupLoads {
             if constexpr (DIM == 1) {
                 const T* p = in + grp0 * 64 + (uint64_t)lane;
 #pragma unroll
-                for (int g = 0; g < G; ++g) x[g] = __builtin_nontemporal_load(p + 64 * g);
+                for (int g = 0; g < G; ++g) x[g] = load_elem<DIM>(p + 64 * g);
             } else {
                 Walker wk;
                 wk.init(grp0, geo);
@@ -805,11 +815,11 @@ struct GroupLoads {
                     // the G blocks are consecutive along x: origins 8 (2D) or 4 (3D) elements apart
                     const T* p = in + block_origin<DIM>(geo, wk) + lg.off;
 #pragma unroll
-                    for (int g = 0; g < G; ++g) x[g] = __builtin_nontemporal_load(p + (DIM == 2 ? 8 : 4) * g);
+                    for (int g = 0; g < G; ++g) x[g] = load_elem<DIM>(p + (DIM == 2 ? 8 : 4) * g);
                 } else {
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
-                        x[g] = __builtin_nontemporal_load(in + block_origin<DIM>(geo, wk) + lg.off);
+                        x[g] = load_elem<DIM>(in + block_origin<DIM>(geo, wk) + lg.off);
                         wk.next(geo);
                     }
                 }
@@ -822,7 +832,7 @@ struct GroupLoads {
             for (int g = 0; g < G; ++g) {
                 uint64_t idx;
                 const bool valid = elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx);
-                x[g] = valid ? __builtin_nontemporal_load(in + idx) : T(0);
+                x[g] = valid ? load_elem<DIM>(in + idx) : T(0);
                 vmask |= (uint64_t)valid << g;
                 if constexpr (DIM != 1) wk.next(geo);
             }
@@ -1065,7 +1075,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             T xv[G];
             const T* p = chunk_ptr(0);
 #pragma unroll
-            for (int g = 0; g < G; ++g) xv[g] = __builtin_nontemporal_load(p + step * g);
+            for (int g = 0; g < G; ++g) xv[g] = load_elem<DIM>(p + step * g);
             for (int c = 0; c < kChunks; ++c) {
                 // the next chunk (the last chunk re-reads itself from cache)
                 const bool more = c + 1 < kChunks;
@@ -1073,7 +1083,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                     if (more) wkn.step(geo, G);
                 }
                 const T* const pn = chunk_ptr(more ? c + 1 : c);
-                process(c, xv, ~0ull, [&](int g) { xv[g] = __builtin_nontemporal_load(pn + step * g); });
+                process(c, xv, ~0ull, [&](int g) { xv[g] = load_elem<DIM>(pn + step * g); });
             }
         } else {
             for (int c = 0; c < kChunks; ++c) {
