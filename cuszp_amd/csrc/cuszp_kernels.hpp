@@ -126,6 +126,8 @@ constexpr int kDppQuadSwap1 = 0xB1;   // quad_perm [1,0,3,2]
 constexpr int kDppQuadSwap2 = 0x4E;   // quad_perm [2,3,0,1]
 constexpr int kDppQuadShr1 = 0x90;    // quad_perm [0,0,1,2]
 constexpr int kDppQuadShr2 = 0x40;    // quad_perm [0,0,0,1]
+constexpr int kDppQuadBcast0 = 0x00;  // quad_perm [0,0,0,0]
+constexpr int kDppRowNewBcast0 = 0x150;   // lane 0 of each row to the whole row
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
@@ -576,13 +578,19 @@ __device__ __forceinline__ uint32_t inverse_lorenzo(uint32_t d, int e) {
         return col + v;
     } else {
         // q(z,y,x) = sum_{z'<=z} d(z',0,0) + sum_{1<=y'<=y} d(z,y',0) + sum_{1<=x'<=x} d(z,y,x')
+        // (lane e = 16z + 4y + x: a DPP row per z, a quad per y)
         const int x = e & 3, y = (e >> 2) & 3;
-        const uint32_t vz = (x == 0 && y == 0) ? d : 0u;
-        const uint32_t vy = (x == 0 && y != 0) ? d : 0u;
         uint32_t vx = x ? d : 0u;
         vx += dpp<kDppQuadShr1>(vx);
         vx += dpp<kDppQuadShr2>(vx);
-        return scan_wave(vz) + scan_row16(vy) + vx;
+        uint32_t vy = (x == 0 && y != 0) ? d : 0u;   // prefix over the quads' first lanes, then across the quad
+        vy += dpp<kDppRowShr4>(vy);
+        vy += dpp<kDppRowShr8>(vy);
+        vy = dpp<kDppQuadBcast0>(vy);
+        uint32_t vz = dpp<kDppRowNewBcast0>((x == 0 && y == 0) ? d : 0u);   // the row's first lane, across the row
+        vz += dpp<kDppBcast15, 0xa>(vz);   // prefix over the rows
+        vz += dpp<kDppBcast31, 0xc>(vz);
+        return vz + vy + vx;
     }
 }
 
