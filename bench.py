@@ -19,6 +19,12 @@ Workloads (--workload):
   1d_plain_f32     16,777,216 f32 sum of 8 sinusoids + noise, 20 % zero runs, plain, rel 1e-3
   1d_plain_f32_4g  2^30 f32 sinusoids + noise (4 GiB), plain, rel 1e-3
   2d_fixed_f64     8192x8192 f64 N(0, 0.02), 2D fixed mode, abs eb 1e-3
+  batch_1000       1,000 arrays of log-uniform sizes 1,024..16,777,216, 70 % f32 /
+                   30 % f64: half dense-layer weights (fixed mode, abs eb 1e-3),
+                   half sinusoid fields (plain/outlier, rel eb 1e-2/1e-3/1e-4).
+                   A step is one batched compress + one batched decompress (one
+                   graph-replayed launch each); the JSON also carries the same
+                   work as one launch per array and the ratio of the two.
 """
 from __future__ import annotations
 
@@ -131,7 +137,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="3d_outlier_f32", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="3d_outlier_f32", choices=sorted(WORKLOADS) + ["batch_1000"])
     ap.add_argument("--cpu-sample-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -155,6 +161,9 @@ def main():
     from cuszp_amd import _lib
 
     L = _lib.load()
+    if args.workload == "batch_1000":
+        run_batch(args, np, torch, L, _lib)
+        return
     wl = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
     tdtype = torch.float32 if wl["dtype"] == "f32" else torch.float64
@@ -288,6 +297,276 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# batched form: 1,000 mixed arrays in one graph-replayed launch per direction
+# ---------------------------------------------------------------------------
+def batch_specs(count=1000, seed=3):
+    """Per-array draws of the mixed batch. Even indices are dense-layer weights
+    (fan_out, fan_in), fans log-uniform in [64, 8192], kept when the element
+    count lies in [1024, 2^24] (redrawn otherwise), processed in 1D fixed mode
+    (2D would need fan_in % 4 == 0) with abs eb 1e-3. Odd indices are 1D
+    sinusoid fields with 20 % zero runs, n log-uniform in [1024, 2^24], plain or
+    outlier mode, rel eb from {1e-2, 1e-3, 1e-4}. 30 % of the arrays are f64."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    lf, hf = math.log(64), math.log(8192)
+    ln, hn = math.log(1024), math.log(1 << 24)
+    out = []
+    for i in range(count):
+        f64 = bool(rng.random() < 0.3)
+        if i % 2 == 0:
+            while True:
+                fo = int(round(math.exp(rng.uniform(lf, hf))))
+                fi = int(round(math.exp(rng.uniform(lf, hf))))
+                if 1024 <= fo * fi <= (1 << 24):
+                    break
+            out.append(dict(kind="weight", n=fo * fi, fan_in=fi, f64=f64, mode=0, abs=1e-3, seed=10_000 + i))
+        else:
+            n = int(round(math.exp(rng.uniform(ln, hn))))
+            out.append(dict(kind="field", n=n, f64=f64, mode=int(rng.integers(1, 3)),
+                            rel=float(rng.choice([1e-2, 1e-3, 1e-4])), seed=10_000 + i))
+    return out
+
+
+def gen_field_batch(torch, n, seed, dtype, dev):
+    """Sum of 8 sinusoids (frequency U(1e-6,1e-3), amplitude U(0.1,1), phase
+    U(0,2pi)) + N(0,1e-3) noise, with runs of U(32,4096) elements zeroed until
+    their lengths add up to 20 % of n (the last run is cut to fit; runs may
+    overlap)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    f, a, ph = rng.uniform(1e-6, 1e-3, 8), rng.uniform(0.1, 1.0, 8), rng.uniform(0, 2 * math.pi, 8)
+    i = torch.arange(n, dtype=torch.float64, device=dev)
+    v = torch.zeros(n, dtype=torch.float64, device=dev)
+    for k in range(8):
+        v += float(a[k]) * torch.sin(2 * math.pi * float(f[k]) * i + float(ph[k]))
+    g = torch.Generator(device=dev).manual_seed(seed)
+    v += torch.randn(n, dtype=torch.float64, device=dev, generator=g) * 1e-3
+    lens = rng.integers(32, 4097, size=int(0.2 * n / 2064 * 1.5) + 16)
+    cs = np.cumsum(lens)
+    k = int(np.searchsorted(cs, int(0.2 * n))) + 1
+    lens = lens[:k].copy()
+    lens[-1] = max(1, lens[-1] - (int(cs[k - 1]) - int(0.2 * n)))   # runs total 20 % of n
+    starts = (rng.random(lens.size) * np.maximum(1, n - lens)).astype(np.int64)
+    ends = np.minimum(starts + lens, n)
+    d = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+    d.index_add_(0, torch.from_numpy(starts).to(dev), torch.ones(lens.size, dtype=torch.int32, device=dev))
+    d.index_add_(0, torch.from_numpy(ends).to(dev), -torch.ones(lens.size, dtype=torch.int32, device=dev))
+    v[torch.cumsum(d, 0)[:n] > 0] = 0
+    return v.to(dtype)
+
+
+def run_batch(args, np, torch, L, _lib):
+    from cuszp_amd import pycuSZp as P
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    t0 = time.time()
+    specs = batch_specs()
+    U0 = _lib.Uint3(0, 0, 0)
+    arrs = []
+    for sp_ in specs:
+        dt = torch.float64 if sp_["f64"] else torch.float32
+        if sp_["kind"] == "weight":
+            g = torch.Generator(device=dev).manual_seed(sp_["seed"])
+            x = (torch.randn(sp_["n"], dtype=torch.float64, device=dev, generator=g)
+                 / math.sqrt(sp_["fan_in"])).to(dt)
+            eb = sp_["abs"]
+        else:
+            x = gen_field_batch(torch, sp_["n"], sp_["seed"], dt, dev)
+            eb = rel_to_abs(torch, x, sp_["rel"])
+        typ = 1 if sp_["f64"] else 0
+        cap = L.cuSZp_max_compressed_size(sp_["n"], 1, U0, typ, sp_["mode"])
+        arrs.append(dict(x=x, n=sp_["n"], eb=eb, typ=typ, mode=sp_["mode"], cap=cap, s=8 if typ else 4,
+                         cb=torch.empty(cap, dtype=torch.uint8, device=dev),
+                         cp=torch.empty(cap, dtype=torch.uint8, device=dev),
+                         yb=torch.empty_like(x), yp=torch.empty_like(x)))
+    torch.cuda.synchronize()
+    raw = sum(a["n"] * a["s"] for a in arrs)
+    log(f"[bench] batch of {len(arrs)} arrays, {raw / 1e9:.3f} GB, generated in {time.time() - t0:.1f}s")
+
+    # one launch per array (the single-array C ABI), sizes to a device array
+    d_sizes = torch.zeros(len(arrs), dtype=torch.int64, device=dev)
+    cfn, dfn = L.cuSZp_compress_async, L.cuSZp_decompress_async
+    cargs = [(a["x"].data_ptr(), a["cp"].data_ptr(), a["cap"], a["n"], d_sizes.data_ptr() + 8 * i, a["eb"], 1, U0,
+              a["typ"], a["mode"], sp) for i, a in enumerate(arrs)]
+
+    def per_array_compress():
+        for c in cargs:
+            if cfn(*c):
+                raise RuntimeError("cuSZp_compress_async failed")
+
+    # batched compress plan, autotuned and graph-captured
+    t0 = time.time()
+    citems = [P.batch_item(a["x"], a["cb"], a["n"], a["cap"], a["eb"], 1, (0, 0, 0), a["typ"], a["mode"])
+              for a in arrs]
+    plan_c = P.BatchPlan(citems, P.BATCH_COMPRESS, graph=True, tune=True, stream=stream)
+    plan_c.execute(stream)
+    sizes, st = plan_c.results(stream)
+    assert all(v == 0 for v in st), st
+    per_array_compress()
+    assert L.cuSZp_last_async_status(sp) == 0
+    sp_sizes = d_sizes.cpu().tolist()
+    assert sizes == sp_sizes, "batched and per-array stream lengths differ"
+    for i, a in enumerate(arrs):
+        assert torch.equal(a["cb"][:sizes[i]], a["cp"][:sizes[i]]), f"array {i}: batched stream differs"
+    C = sum(sizes)
+    dargs = [(a["yp"].data_ptr(), a["cp"].data_ptr(), a["n"], sizes[i], a["eb"], 1, U0, a["typ"], a["mode"], sp)
+             for i, a in enumerate(arrs)]
+
+    def per_array_decompress():
+        for d in dargs:
+            if dfn(*d):
+                raise RuntimeError("cuSZp_decompress_async failed")
+
+    ditems = [P.batch_item(a["cb"], a["yb"], a["n"], sizes[i], a["eb"], 1, (0, 0, 0), a["typ"], a["mode"])
+              for i, a in enumerate(arrs)]
+    plan_d = P.BatchPlan(ditems, P.BATCH_DECOMPRESS, graph=True, tune=True, stream=stream)
+    plan_d.execute(stream)
+    _, st = plan_d.results(stream)
+    assert all(v == 0 for v in st), st
+    per_array_decompress()
+    assert L.cuSZp_last_async_status(sp) == 0
+    max_err = 0.0
+    for i, a in enumerate(arrs):
+        yb, yp = a["yb"], a["yp"]
+        ib = yb.view(torch.int64 if a["typ"] else torch.int32)
+        assert torch.equal(ib, yp.view(ib.dtype)), f"array {i}: batched decompression differs"
+        ebT = float(np.float32(a["eb"])) if a["typ"] == 0 else a["eb"]
+        x = a["x"]
+        fin = torch.isfinite(x) & (x.abs() * (0.5 / ebT) < 2.0**30)
+        max_err = max(max_err, ((x.double() - yb.double()).abs()[fin].max() / ebT).item())
+    log(f"[bench] plans ready and checked in {time.time() - t0:.1f}s: C={C} CR={raw / C:.3f} "
+        f"max_err/eb={max_err:.7f}")
+
+    def timed(fc, fd, steps):
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for e in evs:
+            e[0].record(stream)
+            fc()
+            e[1].record(stream)
+            fd()
+            e[2].record(stream)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t) / steps
+        tc = sum(e[0].elapsed_time(e[1]) for e in evs) / steps * 1e-3
+        td = sum(e[1].elapsed_time(e[2]) for e in evs) / steps * 1e-3
+        return wall, tc, td
+
+    bc = lambda: plan_c.execute(stream)
+    bd = lambda: plan_d.execute(stream)
+    for _ in range(args.warmup):
+        bc()
+        bd()
+        per_array_compress()
+        per_array_decompress()
+    wall_b, tc_b, td_b = timed(bc, bd, args.steps)
+    wall_p, tc_p, td_p = timed(per_array_compress, per_array_decompress, args.steps)
+    assert L.cuSZp_last_async_status(sp) == 0
+    s2, st = plan_c.results(stream)
+    assert s2 == sizes and all(v == 0 for v in st)
+
+    value = 2 * raw / wall_b / 1e9
+    per_array = 2 * raw / wall_p / 1e9
+    dom = "compress" if tc_b >= td_b else "decompress"
+    t_dom = max(tc_b, td_b)
+    algo = raw + C
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_batch_1000.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(dom)
+    cpu = None if args.no_cpu_baseline else cpu_baseline_batch(np, arrs, args.cpu_sample_seconds)
+    nf64 = sum(a["typ"] for a in arrs)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_b * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {
+            "workload": "batch of 1000 arrays, n log-uniform 1024..16777216, 70% f32 / 30% f64; half weights "
+                        "(fixed, abs eb 1e-3), half sinusoid fields (plain/outlier, rel eb 1e-2/1e-3/1e-4); "
+                        "1D; one graph-replayed launch per direction",
+            "name": "batch_1000",
+            "arrays": len(arrs),
+            "f64_arrays": nf64,
+            "elements": sum(a["n"] for a in arrs),
+            "bytes": raw,
+            "compressed_bytes": C,
+            "compression_ratio": round(raw / C, 4),
+            "batched_compress_GBps": round(raw / tc_b / 1e9, 2),
+            "batched_decompress_GBps": round(raw / td_b / 1e9, 2),
+            "batched_compress_ms": round(tc_b * 1e3, 4),
+            "batched_decompress_ms": round(td_b * 1e3, 4),
+            "per_array_GBps": round(per_array, 2),
+            "per_array_compress_GBps": round(raw / tc_p / 1e9, 2),
+            "per_array_decompress_GBps": round(raw / td_p / 1e9, 2),
+            "per_array_ms_per_step": round(wall_p * 1e3, 4),
+            "batched_over_per_array": round(value / per_array, 3),
+            "streams_identical_to_per_array": True,
+            "max_err_over_eb": round(max_err, 8),
+        },
+        "roofline": {
+            "kernel": f"{dom}_batch_kernel",
+            "bound": "hbm",
+            "achieved": round(algo / t_dom / 1e9, 2),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(algo / t_dom / 1e9 / HBM_PEAK_GBPS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": algo,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan_c.close()
+    plan_d.close()
+
+
+def cpu_baseline_batch(np, arrs, budget_s):
+    """Oracle (C++/OpenMP, all host threads) over the batch's arrays in order,
+    compress + decompress each, until about budget_s seconds have been spent."""
+    from oracle import oracle as O
+
+    threads = O.max_threads()
+    tc = td = 0.0
+    raw = 0
+    done = 0
+    for a in arrs:
+        h = a["x"].cpu().numpy()
+        t0 = time.perf_counter()
+        c = O.compress(h, a["eb"], a["mode"], 1, (0, 0, 0), nthreads=threads)
+        t1 = time.perf_counter()
+        O.decompress(c, a["n"], h.dtype, a["eb"], a["mode"], 1, (0, 0, 0), nthreads=threads)
+        tc += t1 - t0
+        td += time.perf_counter() - t1
+        raw += a["n"] * a["s"]
+        done += 1
+        if tc + td > budget_s:
+            break
+    return {
+        "value": round(2 * raw / (tc + td) / 1e9, 4),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {done} arrays of the batch ({raw} bytes), one compress + decompress each",
+        "compress_GBps": round(raw / tc / 1e9, 4),
+        "decompress_GBps": round(raw / td / 1e9, 4),
+    }
 
 
 def copy_calibration(torch, steps):
