@@ -25,6 +25,12 @@ Workloads (--workload):
                    A step is one batched compress + one batched decompress (one
                    graph-replayed launch each); the JSON also carries the same
                    work as one launch per array and the ratio of the two.
+  serve_20000      request server (cuSZp_server_*): 20,000 requests, 60 % compress /
+                   40 % decompress of stored streams, sizes log-uniform 64 KiB..256 MiB,
+                   80 % f32 / 20 % f64, config-3 generators, 64 GB pool admitting by
+                   bytes; seeded Poisson arrivals at 25/50/75/90 % of the server's
+                   closed-loop peak; p50/p99 latency and GB/s beside sequential
+                   per-request single-array calls on the same arrivals.
 """
 from __future__ import annotations
 
@@ -137,7 +143,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="3d_outlier_f32", choices=sorted(WORKLOADS) + ["batch_1000"])
+    ap.add_argument("--workload", default="3d_outlier_f32", choices=sorted(WORKLOADS) + ["batch_1000", "serve_20000"])
     ap.add_argument("--cpu-sample-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -147,7 +153,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the stream shards with an 8-byte exchange per GPU, see DESIGN.md); run with --gpus 1",
+              "(a multi-GPU run would shard the array and exchange one 8-byte stream length per GPU); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
@@ -163,6 +169,9 @@ def main():
     L = _lib.load()
     if args.workload == "batch_1000":
         run_batch(args, np, torch, L, _lib)
+        return
+    if args.workload == "serve_20000":
+        run_serve(args, np, torch, L, _lib)
         return
     wl = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
@@ -566,6 +575,196 @@ def cpu_baseline_batch(np, arrs, budget_s):
         "sample": f"first {done} arrays of the batch ({raw} bytes), one compress + decompress each",
         "compress_GBps": round(raw / tc / 1e9, 4),
         "decompress_GBps": round(raw / td / 1e9, 4),
+    }
+
+
+# ---------------------------------------------------------------------------
+# serving form: Poisson request trace through the request server
+# ---------------------------------------------------------------------------
+def serve_trace(count=20_000, seed=4):
+    """Per-request draws: direction (60 % compress), size in bytes log-uniform in
+    [64 KiB, 256 MiB], f64 with probability 0.2, generator kind (weight or
+    field, half each), field mode and rel eb, and a uniform position inside
+    the source array."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = math.log(64 << 10), math.log(256 << 20)
+    for _ in range(count):
+        comp = bool(rng.random() < 0.6)
+        f64 = bool(rng.random() < 0.2)
+        nbytes = int(math.exp(rng.uniform(lo, hi)))
+        kind = "weight" if rng.random() < 0.5 else "field"
+        mode = 0 if kind == "weight" else int(rng.integers(1, 3))
+        rel = float(rng.choice([1e-2, 1e-3, 1e-4]))
+        out.append(dict(comp=comp, f64=f64, n=max(1, nbytes // (8 if f64 else 4)), kind=kind, mode=mode, rel=rel,
+                        pos=float(rng.random())))
+    return out
+
+
+def run_serve(args, np, torch, L, _lib):
+    from cuszp_amd import pycuSZp as P
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    # sources: one 256 MiB array per (dtype, generator); requests are slices of them
+    src = {}
+    for f64 in (False, True):
+        dt = torch.float64 if f64 else torch.float32
+        n = (256 << 20) // (8 if f64 else 4)
+        g = torch.Generator(device=dev).manual_seed(20 + f64)
+        w = (torch.randn(n, dtype=torch.float64, device=dev, generator=g) / math.sqrt(1024.0)).to(dt)
+        f = gen_field_batch(torch, n, 30 + f64, dt, dev)
+        src[(f64, "weight")] = (w, None)
+        src[(f64, "field")] = (f, (f.max().item(), f.min().item()))
+    trace = serve_trace()
+
+    def eb_of(r):
+        if r["kind"] == "weight":
+            return 1e-3
+        mx, mn = src[(r["f64"], "field")][1]
+        if not r["f64"]:
+            return float(np.float32(r["rel"]) * (np.float32(mx) - np.float32(mn)))
+        return float(np.float32(r["rel"]) * np.float32(mx - mn))
+
+    def slice_of(r):
+        x = src[(r["f64"], r["kind"])][0]
+        st = int(r["pos"] * (x.numel() - r["n"]))
+        return x[st:st + r["n"]]
+
+    # stored streams for the decompress requests: the first 256 compress draws of
+    # another seed, compressed once with the single-array entry
+    store = []
+    for r in serve_trace(count=256, seed=5):
+        x = slice_of(r)
+        eb = eb_of(r)
+        typ = 1 if r["f64"] else 0
+        cap = L.cuSZp_max_compressed_size(r["n"], 1, _lib.Uint3(0, 0, 0), typ, r["mode"])
+        c = torch.empty(cap, dtype=torch.uint8, device=dev)
+        size = P.compress_ex(x, c, cap, r["n"], eb, 1, (0, 0, 0), typ, r["mode"])
+        store.append((c[:size].clone(), size, r["n"], typ, eb, r["mode"]))
+        del c
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(6)
+    reqs, nbytes = [], []
+    for r in trace:
+        if r["comp"]:
+            x = slice_of(r)
+            typ = 1 if r["f64"] else 0
+            reqs.append([P.BATCH_COMPRESS, P.batch_item(x, None, r["n"], 0, eb_of(r), 1, (0, 0, 0), typ, r["mode"])])
+            nbytes.append(r["n"] * (8 if typ else 4))
+        else:
+            c, size, n, typ, eb, mode = store[int(rng.integers(0, len(store)))]
+            reqs.append([P.BATCH_DECOMPRESS, P.batch_item(c, None, n, size, eb, 1, (0, 0, 0), typ, mode)])
+            nbytes.append(n * (8 if typ else 4))
+    total = float(sum(nbytes))
+    mean_b = total / len(reqs)
+    pool = torch.empty(64_000_000_000, dtype=torch.uint8, device=dev)
+    log(f"[bench] serve: {len(reqs)} requests, {total / 1e9:.1f} GB, sources + store ready in {time.time() - t0:.1f}s")
+
+    def pct(v, q):
+        return float(np.percentile(np.asarray(v), q))
+
+    rows = []
+    with P.Server(pool, lanes=3, max_batch=256) as S:
+        def run(arrivals, sequential):
+            rq = [(d, it, t) for (d, it), t in zip(reqs, arrivals)]
+            lat, st, mk = S.replay(rq, sequential=sequential)
+            assert all(v == 0 for v in st), sorted(set(st))
+            return lat, mk
+
+        # warm-up: a short closed-loop run of both forms
+        S.replay([(d, it, 0.0) for d, it in reqs[:200]])
+        S.replay([(d, it, 0.0) for d, it in reqs[:50]], sequential=True)
+        lat, mk = run([0.0] * len(reqs), False)
+        peak = total / mk
+        log(f"[bench] closed-loop peak {peak / 1e9:.1f} GB/s ({mk:.3f} s)")
+        lat_s, mk_s = run([0.0] * len(reqs), True)
+        seq_peak = total / mk_s
+        log(f"[bench] sequential closed-loop {seq_peak / 1e9:.1f} GB/s ({mk_s:.3f} s)")
+        for load in (0.25, 0.5, 0.75, 0.9):
+            lam = load * peak / mean_b
+            arr = np.cumsum(np.random.default_rng(int(load * 100)).exponential(1.0 / lam, len(reqs)))
+            arr = (arr - arr[0]).tolist()
+            lb, mb = run(arr, False)
+            ls, ms = run(arr, True)
+            rows.append({"load": load, "offered_GBps": round(load * peak / 1e9, 1),
+                         "server_GBps": round(total / mb / 1e9, 1),
+                         "server_p50_ms": round(pct(lb, 50) * 1e3, 3), "server_p99_ms": round(pct(lb, 99) * 1e3, 3),
+                         "sequential_GBps": round(total / ms / 1e9, 1),
+                         "sequential_p50_ms": round(pct(ls, 50) * 1e3, 3),
+                         "sequential_p99_ms": round(pct(ls, 99) * 1e3, 3)})
+            log(f"[bench] load {load}: {rows[-1]}")
+    top = rows[-1]
+    cpu = None if args.no_cpu_baseline else cpu_baseline_serve(np, reqs, trace, slice_of, store, eb_of, rng,
+                                                               args.cpu_sample_seconds)
+    out = {
+        "metric": METRIC,
+        "value": top["server_GBps"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": len(rows),
+        "warmup": 1,
+        "ms_per_step": round(total / (top["server_GBps"] * 1e9) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {
+            "workload": "request server: 20000 requests (60% compress / 40% decompress of 256 stored streams), "
+                        "64 KiB..256 MiB log-uniform, 80% f32 / 20% f64, weights (fixed, abs 1e-3) and sinusoid "
+                        "fields (plain/outlier, rel 1e-2/1e-3/1e-4), 1D; 64 GB pool, 3 lanes, batches of up to 256; "
+                        "Poisson arrivals at 25/50/75/90% of the closed-loop peak",
+            "name": "serve_20000",
+            "requests": len(reqs),
+            "bytes": int(total),
+            "closed_loop_peak_GBps": round(peak / 1e9, 1),
+            "sequential_closed_loop_GBps": round(seq_peak / 1e9, 1),
+            "value_is": "server GB/s at 90% offered load",
+            "p99_ms_at_90pct": top["server_p99_ms"],
+            "loads": rows,
+        },
+        "roofline": {
+            "kernel": "server, closed loop (all batched kernels and launch gaps)",
+            "bound": "hbm",
+            "achieved": round(peak / 1e9, 2),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(peak / 1e9 / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "note": "uncompressed bytes per second of the closed-loop run; streams add about 1/CR on top",
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_serve(np, reqs, trace, slice_of, store, eb_of, rng, budget_s):
+    """Oracle (C++/OpenMP, all host threads) over the trace's compress requests
+    in order, until about budget_s seconds have been spent."""
+    from oracle import oracle as O
+
+    threads = O.max_threads()
+    t_all, raw, done = 0.0, 0, 0
+    for r in trace:
+        if not r["comp"]:
+            continue
+        h = slice_of(r).cpu().numpy()
+        t0 = time.perf_counter()
+        O.compress(h, eb_of(r), r["mode"], 1, (0, 0, 0), nthreads=threads)
+        t_all += time.perf_counter() - t0
+        raw += h.nbytes
+        done += 1
+        if t_all > budget_s:
+            break
+    return {
+        "value": round(raw / t_all / 1e9, 4),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"the trace's first {done} compress requests ({raw} bytes), compress only",
     }
 
 

@@ -45,6 +45,13 @@ EXPORTED = (
         "cuSZp_batch_plan_destroy",
         "cuSZp_compress_batch",
         "cuSZp_decompress_batch",
+        "cuSZp_server_create",
+        "cuSZp_server_submit",
+        "cuSZp_server_poll",
+        "cuSZp_server_release",
+        "cuSZp_server_stats",
+        "cuSZp_server_destroy",
+        "cuSZp_server_replay",
     ]
 )
 
@@ -61,6 +68,25 @@ class BatchItem(ctypes.Structure):
         ("type", ctypes.c_int),
         ("mode", ctypes.c_int),
     ]
+
+
+class Completion(ctypes.Structure):
+    """cuszp_completion_t (include/cuSZp.h, serving form)."""
+
+    _fields_ = [
+        ("id", ctypes.c_ulonglong),
+        ("status", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("offset", ctypes.c_size_t),
+        ("d_out", ctypes.c_void_p),
+        ("bytes", ctypes.c_size_t),
+    ]
+
+
+class Request(ctypes.Structure):
+    """cuszp_request_t (include/cuSZp.h, load replay)."""
+
+    _fields_ = [("direction", ctypes.c_int), ("item", BatchItem), ("arrival_s", ctypes.c_double)]
 
 
 BATCH_COMPRESS, BATCH_DECOMPRESS = 0, 1
@@ -128,5 +154,20 @@ def load() -> ctypes.CDLL:
     L.cuSZp_compress_batch.restype = i32
     L.cuSZp_decompress_batch.argtypes = [pitems, sz, ctypes.POINTER(i32), vp]
     L.cuSZp_decompress_batch.restype = i32
+    pd = ctypes.POINTER(ctypes.c_double)
+    L.cuSZp_server_create.argtypes = [ctypes.POINTER(vp), vp, sz, i32, sz]
+    L.cuSZp_server_create.restype = i32
+    L.cuSZp_server_submit.argtypes = [vp, i32, pitems, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.cuSZp_server_submit.restype = i32
+    L.cuSZp_server_poll.argtypes = [vp, ctypes.POINTER(Completion), sz, psz, i32]
+    L.cuSZp_server_poll.restype = i32
+    L.cuSZp_server_release.argtypes = [vp, ctypes.c_ulonglong]
+    L.cuSZp_server_release.restype = i32
+    L.cuSZp_server_stats.argtypes = [vp, psz, psz, psz]
+    L.cuSZp_server_stats.restype = i32
+    L.cuSZp_server_destroy.argtypes = [vp]
+    L.cuSZp_server_destroy.restype = None
+    L.cuSZp_server_replay.argtypes = [vp, ctypes.POINTER(Request), sz, i32, pd, ctypes.POINTER(i32), pd]
+    L.cuSZp_server_replay.restype = i32
     _lib = L
     return L

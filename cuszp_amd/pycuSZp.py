@@ -282,3 +282,106 @@ def compress_tensors(tensors, error_bounds, modes, dims_list=None, dims_kind=Non
         plan.execute(stream)
         sizes, _ = plan.results(stream)
     return list(zip(outs, sizes))
+
+
+# ---------------------------------------------------------------------------
+# serving form: a request server with continuous batching (include/cuSZp.h,
+# cuSZp_server_*). Outputs are allocated in a caller-provided device pool.
+# ---------------------------------------------------------------------------
+from ._lib import Completion, Request  # noqa: E402
+
+
+class Server:
+    """Owns a cuszp_server_t over `pool` (a contiguous CUDA uint8 tensor, kept
+    alive by the server). `submit_*` return a request id; `poll()` returns
+    completed requests as (id, status, direction, offset, nbytes) tuples, whose
+    output is pool[offset:offset + nbytes] until `release(id)`."""
+
+    def __init__(self, pool, lanes: int = 3, max_batch: int = 256):
+        if not pool.is_contiguous():
+            raise ValueError("pool must be contiguous")
+        self.pool = pool
+        self._s = ctypes.c_void_p(0)
+        rc = lib.cuSZp_server_create(ctypes.byref(self._s), pool.data_ptr(), pool.numel() * pool.element_size(),
+                                     lanes, max_batch)
+        if rc:
+            raise CuSZpError(rc, "server create")
+        self._buf = (Completion * 256)()
+
+    def submit(self, direction: int, item) -> int:
+        rid = ctypes.c_ulonglong(0)
+        rc = lib.cuSZp_server_submit(self._s, direction, ctypes.byref(item), ctypes.byref(rid))
+        if rc:
+            raise CuSZpError(rc, "server submit")
+        return rid.value
+
+    def submit_compress(self, t, error_bound: float, mode=1, dim: int = 1, dims=(0, 0, 0)) -> int:
+        mi = MODES[mode] if isinstance(mode, str) else int(mode)
+        return self.submit(BATCH_COMPRESS, batch_item(t if t.numel() else None, None, t.numel(), 0, error_bound,
+                                                      dim, tuple(dims), _type_of(t), mi))
+
+    def submit_decompress(self, cmp, cmp_size: int, num_elements: int, dtype, error_bound: float, mode=1,
+                          dim: int = 1, dims=(0, 0, 0)) -> int:
+        import torch
+
+        mi = MODES[mode] if isinstance(mode, str) else int(mode)
+        typ = CUSZP_TYPE_FLOAT if dtype == torch.float32 else CUSZP_TYPE_DOUBLE
+        return self.submit(BATCH_DECOMPRESS, batch_item(cmp if num_elements else None, None, num_elements, cmp_size,
+                                                        error_bound, dim, tuple(dims), typ, mi))
+
+    def poll(self, wait: bool = False):
+        n = ctypes.c_size_t(0)
+        out = []
+        while True:
+            rc = lib.cuSZp_server_poll(self._s, self._buf, len(self._buf), ctypes.byref(n), 1 if wait and not out else 0)
+            if rc:
+                raise CuSZpError(rc, "server poll")
+            out += [(c.id, c.status, c.direction, c.offset, c.bytes) for c in self._buf[: n.value]]
+            if n.value < len(self._buf):
+                return out
+
+    def release(self, rid: int) -> None:
+        rc = lib.cuSZp_server_release(self._s, rid)
+        if rc:
+            raise CuSZpError(rc, "server release")
+
+    def stats(self):
+        q, f, u = ctypes.c_size_t(0), ctypes.c_size_t(0), ctypes.c_size_t(0)
+        rc = lib.cuSZp_server_stats(self._s, ctypes.byref(q), ctypes.byref(f), ctypes.byref(u))
+        if rc:
+            raise CuSZpError(rc, "server stats")
+        return q.value, f.value, u.value
+
+    def replay(self, requests, sequential: bool = False):
+        """requests: [(direction, BatchItem, arrival_s)] with non-decreasing
+        arrivals. Returns (latencies_s, statuses, makespan_s)."""
+        n = len(requests)
+        arr = (Request * max(n, 1))()
+        for i, (d, it, t) in enumerate(requests):
+            arr[i].direction = d
+            arr[i].item = it
+            arr[i].arrival_s = t
+        lat = (ctypes.c_double * max(n, 1))()
+        st = (ctypes.c_int * max(n, 1))()
+        mk = ctypes.c_double(0)
+        rc = lib.cuSZp_server_replay(self._s, arr, n, 1 if sequential else 0, lat, st, ctypes.byref(mk))
+        if rc:
+            raise CuSZpError(rc, "server replay")
+        return list(lat)[:n], list(st)[:n], mk.value
+
+    def close(self) -> None:
+        if self._s:
+            lib.cuSZp_server_destroy(self._s)
+            self._s = ctypes.c_void_p(0)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -10,7 +10,7 @@
  * pointers are plain HIP device pointers (hipMalloc / PyTorch-ROCm tensors);
  * `stream` is a hipStream_t (0 = null stream).
  *
- * Differences from the reference, all deliberate and documented in DESIGN.md:
+ * Differences from the reference, all deliberate:
  *   - dims is passed as the POD `cuszp_dims_t` {x, y, z} (x fastest), which has
  *     the layout of CUDA's uint3 and of pycuSZp's ctypes Uint3;
  *   - *cmpSize is the true stream length (the reference under-reports it for
@@ -208,6 +208,60 @@ void cuSZp_batch_plan_destroy(cuszp_batch_plan_t plan);
 int cuSZp_compress_batch(const cuszp_batch_item_t* items, size_t count, size_t* cmpSizes, int* statuses,
                          hipStream_t stream);
 int cuSZp_decompress_batch(const cuszp_batch_item_t* items, size_t count, int* statuses, hipStream_t stream);
+
+/* ---- serving form (no reference counterpart) ----
+ * A request server with continuous batching. Requests are compress or
+ * decompress operations on single arrays, submitted as they arrive; each
+ * result is identical to the single-array call's. Outputs live in a device
+ * pool the caller provides (sub-allocated in 256-byte units); admission is
+ * first come, first served, by output bytes: the oldest waiting request starts
+ * as soon as its output fits in the pool. Admitted requests run in batches on
+ * up to `lanes` streams at once (one batched launch per direction per batch),
+ * so requests that arrive while a batch runs start on another lane without
+ * waiting for it. A completed request keeps its output in the pool until
+ * cuSZp_server_release. All calls on one server are serialised internally. */
+typedef struct cuszp_server_s* cuszp_server_t;
+typedef struct {
+    unsigned long long id;  /* as returned by cuSZp_server_submit */
+    int status;             /* CUSZP_SUCCESS or the request's error */
+    int direction;          /* CUSZP_BATCH_COMPRESS / CUSZP_BATCH_DECOMPRESS */
+    size_t offset;          /* byte offset of the output in the pool */
+    void* d_out;            /* pool + offset */
+    size_t bytes;           /* compress: stream length; decompress: nbEle * sizeof(value) */
+} cuszp_completion_t;
+/* lanes: 1..4 concurrent batches; max_batch: requests per batch (1..2^20). */
+int cuSZp_server_create(cuszp_server_t* server, void* d_pool, size_t pool_bytes, int lanes, size_t max_batch);
+/* item->d_out is ignored (the output is allocated in the pool); for
+ * decompression item->capacity is the stream length. Validation errors are
+ * returned here (no id is issued); requests whose output exceeds the pool
+ * return CUSZP_ERROR_TOO_LARGE. Launches what it can before returning. */
+int cuSZp_server_submit(cuszp_server_t server, int direction, const cuszp_batch_item_t* item,
+                        unsigned long long* id);
+/* Retires finished batches, admits and launches waiting requests, and copies
+ * up to max_done completions (oldest first) to done[]. With wait != 0 it
+ * blocks until at least one completion is available or nothing is running. */
+int cuSZp_server_poll(cuszp_server_t server, cuszp_completion_t* done, size_t max_done, size_t* n_done, int wait);
+/* Returns a completed request's output bytes to the pool. */
+int cuSZp_server_release(cuszp_server_t server, unsigned long long id);
+/* Requests waiting for pool space, admitted or running, and pool bytes held. */
+int cuSZp_server_stats(cuszp_server_t server, size_t* queued, size_t* in_flight, size_t* pool_used);
+/* Waits for running batches and frees the server's own resources (not the pool). */
+void cuSZp_server_destroy(cuszp_server_t server);
+
+/* Load replay for measuring a server: request i is submitted at arrival_s
+ * seconds after the call starts (arrival times non-decreasing), every
+ * completion is released at once, and latency_s[i] = completion seen -
+ * arrival. sequential != 0 instead runs the requests one at a time in arrival
+ * order through cuSZp_compress_ex / cuSZp_decompress_ex (output at the start
+ * of the pool; the server must hold no requests). *makespan_s = time until the
+ * last completion. */
+typedef struct {
+    int direction;
+    cuszp_batch_item_t item;
+    double arrival_s;
+} cuszp_request_t;
+int cuSZp_server_replay(cuszp_server_t server, const cuszp_request_t* requests, size_t count, int sequential,
+                        double* latency_s, int* statuses, double* makespan_s);
 
 /* Library version string ("cuszp_amd <semver> gfx950"). */
 const char* cuSZp_version(void);

@@ -133,3 +133,27 @@ def test_batch_plan_rejects_before_device_work(L):
     plan = ctypes.c_void_p(0)
     assert L.cuSZp_batch_plan_create(ctypes.byref(plan), arr, 1, 1, 0, None, None) == 5
     assert L.cuSZp_batch_plan_create(ctypes.byref(plan), arr, 1, 7, 0, None, None) == 1   # bad direction
+
+
+def test_server_struct_layouts():
+    C, R = _lib.Completion, _lib.Request
+    assert ctypes.sizeof(C) == 40
+    assert [C.status.offset, C.direction.offset, C.offset.offset, C.d_out.offset, C.bytes.offset] == [8, 12, 16, 24, 32]
+    assert ctypes.sizeof(R) == 80 and R.item.offset == 8 and R.arrival_s.offset == 72
+
+
+def test_server_rejects_before_device_work(L):
+    # argument checks run before any HIP call: no GPU needed
+    s = ctypes.c_void_p(0)
+    assert L.cuSZp_server_create(ctypes.byref(s), None, 1 << 20, 2, 16) == 1        # no pool
+    assert L.cuSZp_server_create(ctypes.byref(s), 4096, 1 << 20, 0, 16) == 1        # no lanes
+    assert L.cuSZp_server_create(ctypes.byref(s), 4096, 1 << 20, 5, 16) == 1        # too many lanes
+    assert L.cuSZp_server_create(ctypes.byref(s), 4096, 1 << 20, 2, 0) == 1         # empty batches
+    assert L.cuSZp_server_create(ctypes.byref(s), 4096, 100, 2, 16) == 1            # pool below one unit
+    assert not s.value
+    rid = ctypes.c_ulonglong(0)
+    it = _lib.BatchItem(16, 16, 1000, 0, 1e-3, 1, _lib.Uint3(0, 0, 0), 0, 1)
+    assert L.cuSZp_server_submit(None, 0, ctypes.byref(it), ctypes.byref(rid)) == 1
+    n = ctypes.c_size_t(0)
+    assert L.cuSZp_server_poll(None, None, 0, ctypes.byref(n), 0) == 1
+    assert L.cuSZp_server_release(None, 1) == 1
