@@ -769,6 +769,9 @@ constexpr uint32_t kStageBytes = 6144;
 // Waves per SIMD the compress kernels are built for (the register budget):
 // six 4-wave workgroups of about 26 KB of LDS fit one CU's 160 KB.
 constexpr int kCompressWavesPerSimd = 6;
+// Waves per SIMD the decompress kernels are built for: 8 needs at most 64
+// VGPRs and few enough SGPRs for eight waves' scalar files.
+constexpr int kDecompressWavesPerSimd = 8;
 
 // Shared memory of a compress workgroup.
 template <int WAVES>
@@ -1094,10 +1097,20 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                 process(c, xv, ~0ull, [&](int g) { xv[g] = load_elem<DIM>(pn + step * g); });
             }
         } else {
+            // groups holding elements of the array; a chunk past them is all
+            // zero: rate bytes 0 in every mode and no payload, so it only writes
+            // its header bytes (1D header padding, 2D/3D pad bytes below H)
+            const uint64_t gvalid = (DIM == 1) ? (geo.n + 63) / 64 : geo.nblocks;
             for (int c = 0; c < kChunks; ++c) {
+                const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
+                if (grp0 >= gvalid) {
+                    const uint64_t b0 = (DIM == 1) ? 2 * grp0 : grp0;
+                    if (!direct && lane < NB && (DIM == 1 || b0 + (uint64_t)lane < geo.H)) dst[b0 + (uint64_t)lane] = 0;
+                    continue;
+                }
                 T xv[G];
                 uint64_t vmask;
-                GroupLoads<DIM, T, G>::template run<false>(in, geo, lg, wgrp0 + (uint64_t)c * G, lane, xv, vmask);
+                GroupLoads<DIM, T, G>::template run<false>(in, geo, lg, grp0, lane, xv, vmask);
                 process(c, xv, vmask, [](int) {});
             }
         }
@@ -1170,6 +1183,9 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
     const Geom& geo = cur.geo;
     const uint8_t* __restrict__ cmp = (const uint8_t*)cur.src;
+    // a word that is always 0: the guard words in front of the status array are
+    // never written (lanes without a stream word load it, so nothing needs zeroing)
+    const uint8_t* const zw = (const uint8_t*)(sc.status - 1);
     const uint64_t lt = tile - cur.tile0;
     const uint64_t wgrp0 = (lt * WAVES + (uint64_t)wave) * (kChunks * G);   // this wave's first group
     const uint64_t wb0 = (DIM == 1) ? 2 * wgrp0 : wgrp0;                     // and first block
@@ -1247,20 +1263,20 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         }
         return rdlane(incl, kWaveSize - 1);
     };
-    // stream word of this lane in group g of the chunk in slot s, as loaded: a
-    // lane without a word loads the stream's first word (no branch around the
-    // load, so the compiler's waits stay exact), which decode() zeroes at the use
+    // stream word of this lane in group g of the chunk in slot s: a lane without
+    // a word (or any lane of a truncated stream) loads the zero word, so there
+    // is no branch around the load and nothing to mask at the use
     auto word = [&](int s, int g) -> uint32_t {
         const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
         const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
         const bool has = intact && m.has(l);
-        return load_u32u(has ? wbase + (m.off() + (wideg ? wo_wide : wo)) : cmp);
+        return load_u32u(has ? wbase + (m.off() + (wideg ? wo_wide : wo)) : zw);
     };
     // decodes group g of the chunk in slot s from its loaded word w
     auto decode = [&](int s, int g, uint32_t w, auto widec) -> T {
         constexpr bool WIDE = decltype(widec)::value;   // 2D/3D 32-plane blocks possible
         const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
-        const uint32_t x = xpose32((intact && m.has(l)) ? w : 0u, xc);
+        const uint32_t x = xpose32(w, xc);
         uint32_t a = x & 0x7fffffffu, neg = x >> 31;   // lane l: sign in bit 31 over the magnitude bits
         if constexpr (WIDE) {
             if (m.wide()) {   // 32 planes: the sign words sit in their own two words
@@ -1269,16 +1285,17 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
                 a = x;
             }
         }
-        if constexpr (MODE == kOutlier) {
-            if (m.sel() && l == 7 && (DIM == 1 || h == 0)) a = sm.aux[s][wave][BPG * g + (DIM == 1 ? h : 0)];   // element 0
+        if constexpr (MODE == kOutlier) {   // element 0 of a selected block (0 when not intact)
+            const uint32_t e0 = sm.aux[s][wave][BPG * g + (DIM == 1 ? h : 0)];
+            a = (m.sel() && l == 7 && (DIM == 1 || h == 0)) ? e0 : a;
         }
         const uint32_t dm = neg ? 0u - a : a;
         const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
         uint32_t q;
         if constexpr (MODE == kFixed) q = d;
         else q = inverse_lorenzo<DIM>(d, lane);
-        const bool zero_blk = !intact || !m.nz();
-        return zero_blk ? T(0) : (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
+        // a block without payload (or of a truncated stream) decoded all-zero words: q = 0
+        return (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
     };
 
     // 2D/3D blocks of 32 planes (not in outlier streams) take the general loop
@@ -1331,7 +1348,9 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         Walker wk;
         if constexpr (DIM != 1) wk.init(wgrp0, geo);
         uint32_t coff = 0;
+        const uint64_t gvalid = (DIM == 1) ? (geo.n + 63) / 64 : geo.nblocks;   // groups holding elements
         for (int c = 0; c < kChunks; ++c) {
+            if (wgrp0 + (uint64_t)c * G >= gvalid) break;   // the rest of the wave writes nothing
             coff += bookkeep(c, coff, 0);
             __builtin_amdgcn_wave_barrier();
             uint32_t W[G];
@@ -1363,7 +1382,7 @@ compress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
 }
 
 template <int DIM, int MODE, class T, int WAVES>
-__global__ void __launch_bounds__(WAVES * kWaveSize)
+__global__ void __launch_bounds__(WAVES * kWaveSize, kDecompressWavesPerSimd)
 decompress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
     __shared__ SmemD<WAVES> sm;
     decompress_tile<DIM, MODE, T, WAVES>(item, base + blockIdx.x, sc, sm);
@@ -1412,7 +1431,7 @@ compress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx s
 }
 
 template <int WAVES>
-__global__ void __launch_bounds__(WAVES * kWaveSize)
+__global__ void __launch_bounds__(WAVES * kWaveSize, kDecompressWavesPerSimd)
 decompress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx sc,
                         const unsigned long long* __restrict__ epoch, const uint64_t base) {
     __shared__ SmemD<WAVES> sm;
