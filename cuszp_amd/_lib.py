@@ -52,6 +52,8 @@ EXPORTED = (
         "cuSZp_server_stats",
         "cuSZp_server_destroy",
         "cuSZp_server_replay",
+        "cuSZp_value_range",
+        "cuSZp_rel_error_bound",
     ]
 )
 
@@ -169,5 +171,9 @@ def load() -> ctypes.CDLL:
     L.cuSZp_server_destroy.restype = None
     L.cuSZp_server_replay.argtypes = [vp, ctypes.POINTER(Request), sz, i32, pd, ctypes.POINTER(i32), pd]
     L.cuSZp_server_replay.restype = i32
+    L.cuSZp_value_range.argtypes = [vp, sz, i32, ctypes.POINTER(f64), ctypes.POINTER(f64), vp]
+    L.cuSZp_value_range.restype = i32
+    L.cuSZp_rel_error_bound.argtypes = [vp, sz, i32, f32, ctypes.POINTER(f32), vp]
+    L.cuSZp_rel_error_bound.restype = i32
     _lib = L
     return L

@@ -128,16 +128,37 @@ def _type_of(t) -> int:
     raise TypeError(f"cuSZp compresses float32/float64 tensors, got {t.dtype}")
 
 
-def rel_to_abs(t, rel_eb: float) -> float:
-    """Relative -> absolute error bound like examples/cuSZp.cpp:176-210:
-    f32: (float)rel * (float)(max - min) in float; f64: rel * (float)(max - min)."""
-    import numpy as np
+def value_range(t, stream=None) -> Tuple[float, float]:
+    """(min, max) of a contiguous float32/float64 CUDA tensor, computed on the
+    device as the reference CLI's loop does (examples/cuSZp.cpp:176-210): NaNs
+    are skipped unless element 0 is NaN. Synchronises the stream."""
+    if not t.is_contiguous() or t.numel() == 0:
+        raise ValueError("tensor must be contiguous and non-empty")
+    mn, mx = ctypes.c_double(0), ctypes.c_double(0)
+    rc = lib.cuSZp_value_range(_ptr(t), t.numel(), _type_of(t), ctypes.byref(mn), ctypes.byref(mx), _stream(stream))
+    if rc:
+        raise CuSZpError(rc, "value range")
+    return mn.value, mx.value
 
-    mx = t.max().item()
-    mn = t.min().item()
-    if t.dtype.itemsize == 4:
-        return float(np.float32(rel_eb) * (np.float32(mx) - np.float32(mn)))
-    return float(np.float32(rel_eb) * np.float32(mx - mn))
+
+def rel_to_abs(t, rel_eb: float, stream=None) -> float:
+    """Relative -> absolute error bound like examples/cuSZp.cpp:176-210, on the
+    device: f32: (float)rel * (float)(max - min) in float; f64: rel * (float)(max - min)."""
+    if not t.is_contiguous() or t.numel() == 0:
+        raise ValueError("tensor must be contiguous and non-empty")
+    eb = ctypes.c_float(0)
+    rc = lib.cuSZp_rel_error_bound(_ptr(t), t.numel(), _type_of(t), float(rel_eb), ctypes.byref(eb), _stream(stream))
+    if rc:
+        raise CuSZpError(rc, "relative error bound")
+    return eb.value
+
+
+def compress_tensor_rel(t, rel_eb: float, mode="plain", dim: int = 1, dims: Optional[Tuple[int, int, int]] = None,
+                        stream=None):
+    """compress_tensor with a relative bound: returns (uint8 tensor, size, absolute eb)."""
+    eb = rel_to_abs(t, rel_eb, stream)
+    out, n = compress_tensor(t, eb, mode, dim, dims, stream)
+    return out, n, eb
 
 
 def compress_tensor(t, error_bound: float, mode="plain", dim: int = 1, dims: Optional[Tuple[int, int, int]] = None,

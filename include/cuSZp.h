@@ -263,6 +263,19 @@ typedef struct {
 int cuSZp_server_replay(cuszp_server_t server, const cuszp_request_t* requests, size_t count, int sequential,
                         double* latency_s, int* statuses, double* makespan_s);
 
+/* ---- relative error bounds (the step before the path) ----
+ * The reference CLI turns a relative bound into an absolute one on the host
+ * (examples/cuSZp.cpp:176-210 of the reference); these compute the same on the
+ * device array. cuSZp_value_range: min and max of the elements, NaNs skipped
+ * unless element 0 is NaN (then both are NaN), as the CLI's comparison loop
+ * gives. cuSZp_rel_error_bound: *errorBound = rel * (max - min) in float
+ * arithmetic (f64 arrays: the difference in double, rounded to float). Both
+ * synchronise `stream`. */
+int cuSZp_value_range(const void* d_data, size_t nbEle, cuszp_type_t type, double* min_val, double* max_val,
+                      hipStream_t stream);
+int cuSZp_rel_error_bound(const void* d_data, size_t nbEle, cuszp_type_t type, float rel, float* errorBound,
+                          hipStream_t stream);
+
 /* Library version string ("cuszp_amd <semver> gfx950"). */
 const char* cuSZp_version(void);
 

@@ -24,6 +24,7 @@
 //   header size 2D/3D ....... src/cuSZp_kernels_2D_f32.cu:46 (roundup4(blockNum))
 //   block counts ............ src/cuSZp_entry_2D_f32.cu:21-23, src/cuSZp_entry_3D_f32.cu:21-24
 //   decoded value ........... src/cuSZp_kernels_1D_f32.cu:490 (currQuant * eb * 2)
+//   relative bound .......... examples/cuSZp.cpp:176-210 (value range loop, float product)
 //
 // Decided semantics where the reference is ambiguous or buggy (each pinned by a
 // test in tests/):
@@ -386,6 +387,34 @@ int cuszp_oracle_decompress(void* out, int type, int dim, int mode, size_t n,
     if (type == TYPE_F32)
         return decompress_t<float>(g, mode, (float*)out, (float)eb, cmp, cmp_size, nthreads);
     return decompress_t<double>(g, mode, (double*)out, eb, cmp, cmp_size, nthreads);
+}
+
+// Relative error bound as the reference CLI computes it, examples/cuSZp.cpp:176-210:
+// one pass from element 0 keeping max, and min only when the element is not
+// above max; then rel * (max - min) in float (f64: the difference in double,
+// rounded to float). Writes min and max; returns the absolute bound.
+float cuszp_oracle_rel_eb(const void* data, int type, size_t n, float rel, double* mn, double* mx) {
+    if (!n) return 0.0f;
+    if (type == TYPE_F32) {
+        const float* x = (const float*)data;
+        float hi = x[0], lo = x[0];
+        for (size_t i = 0; i < n; i++) {
+            if (x[i] > hi) hi = x[i];
+            else if (x[i] < lo) lo = x[i];
+        }
+        *mn = lo;
+        *mx = hi;
+        return rel * (hi - lo);
+    }
+    const double* x = (const double*)data;
+    double hi = x[0], lo = x[0];
+    for (size_t i = 0; i < n; i++) {
+        if (x[i] > hi) hi = x[i];
+        else if (x[i] < lo) lo = x[i];
+    }
+    *mn = lo;
+    *mx = hi;
+    return rel * (float)(hi - lo);
 }
 
 int cuszp_oracle_max_threads(void) {

@@ -50,6 +50,8 @@ def lib():
         L.cuszp_oracle_compress.restype = ctypes.c_longlong
         L.cuszp_oracle_decompress.argtypes = [vp, i32, i32, i32, sz, u32, u32, u32, dbl, vp, sz, i32]
         L.cuszp_oracle_decompress.restype = i32
+        L.cuszp_oracle_rel_eb.argtypes = [vp, i32, sz, ctypes.c_float, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.cuszp_oracle_rel_eb.restype = ctypes.c_float
         L.cuszp_oracle_max_threads.argtypes = []
         L.cuszp_oracle_max_threads.restype = i32
         _lib = L
@@ -80,6 +82,15 @@ def num_blocks(dim, n, dims=(0, 0, 0)):
 
 def max_size(dim, n, dims=(0, 0, 0)):
     return lib().cuszp_oracle_max_size(dim, n, *_dims(dim, n, dims))
+
+
+def rel_eb(data: np.ndarray, rel: float):
+    """(abs eb, min, max) as the reference CLI derives a relative bound (examples/cuSZp.cpp:176-210)."""
+    data = np.ascontiguousarray(data)
+    typ = 0 if data.dtype == np.float32 else 1
+    mn, mx = ctypes.c_double(0), ctypes.c_double(0)
+    eb = lib().cuszp_oracle_rel_eb(data.ctypes.data, typ, data.size, rel, ctypes.byref(mn), ctypes.byref(mx))
+    return float(eb), mn.value, mx.value
 
 
 def max_threads():

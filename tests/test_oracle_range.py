@@ -1,0 +1,38 @@
+"""The oracle's relative error bound (examples/cuSZp.cpp:176-210 of the
+reference), pinned against numpy: the CLI's comparison loop equals
+nanmin/nanmax whenever element 0 is not NaN, the bound is the float product
+rel * (max - min) (f64: the difference rounded to float first), and a NaN in
+element 0 makes everything NaN."""
+import numpy as np
+import pytest
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("seed", range(4))
+def test_rel_eb_matches_numpy(oracle, dtype, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 50_000))
+    x = (rng.standard_normal(n) * 10.0 ** rng.uniform(-3, 3)).astype(dtype)
+    if n > 10:
+        x[rng.integers(1, n, 5)] = np.nan   # never element 0
+    eb, mn, mx = oracle.rel_eb(x, 1e-3)
+    assert mn == np.nanmin(x) and mx == np.nanmax(x)
+    if dtype == np.float32:
+        want = np.float32(1e-3) * (np.float32(mx) - np.float32(mn))
+    else:
+        want = np.float32(1e-3) * np.float32(np.float64(mx) - np.float64(mn))
+    assert np.float32(eb) == want
+
+
+def test_rel_eb_special_values(oracle):
+    x = np.array([2.0, -np.inf, 1.0], np.float32)
+    assert oracle.rel_eb(x, 0.5)[1:] == (-np.inf, 2.0)
+    x = np.array([np.nan, 1.0, 2.0], np.float32)
+    eb, mn, mx = oracle.rel_eb(x, 0.5)
+    assert np.isnan(eb) and np.isnan(mn) and np.isnan(mx)
+    x = np.array([7.0], np.float64)
+    assert oracle.rel_eb(x, 0.5) == (0.0, 7.0, 7.0)
+    # the f32 difference is taken in float: 16777217 is not representable
+    x = np.array([0.0, 16777216.0, -1.0], np.float32)
+    eb, mn, mx = oracle.rel_eb(x, 1.0)
+    assert eb == float(np.float32(16777216.0) - np.float32(-1.0)) == 16777216.0
