@@ -54,6 +54,7 @@ EXPORTED = (
         "cuSZp_server_replay",
         "cuSZp_value_range",
         "cuSZp_rel_error_bound",
+        "cuSZp_quality",
     ]
 )
 
@@ -175,5 +176,7 @@ def load() -> ctypes.CDLL:
     L.cuSZp_value_range.restype = i32
     L.cuSZp_rel_error_bound.argtypes = [vp, sz, i32, f32, ctypes.POINTER(f32), vp]
     L.cuSZp_rel_error_bound.restype = i32
+    L.cuSZp_quality.argtypes = [vp, vp, sz, i32, ctypes.POINTER(f64), vp]
+    L.cuSZp_quality.restype = i32
     _lib = L
     return L

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcuszp_amd.so")
-SOURCES = ["kern_batch_c.hip", "kern_batch_d.hip", "kern_f32.hip", "kern_f64.hip", "kern_range.hip", "cuszp_api.cpp",
+SOURCES = ["kern_batch_c.hip", "kern_batch_d.hip", "kern_f32.hip", "kern_f64.hip", "kern_range.hip", "kern_quality.hip",
+           "cuszp_api.cpp",
            "cuszp_batch.cpp", "cuszp_serve.cpp"]
 ARCH = os.environ.get("CUSZP_OFFLOAD_ARCH", "gfx950")
 

@@ -153,6 +153,25 @@ def rel_to_abs(t, rel_eb: float, stream=None) -> float:
     return eb.value
 
 
+QUALITY_FIELDS = ("mse", "range", "psnr", "norm_err", "norm_err_norm", "nrmse", "max_abs_err", "max_rel_err",
+                  "correlation")
+
+
+def quality(ori, dec, stream=None) -> dict:
+    """Reconstruction quality of `dec` against `ori` (same dtype and size,
+    contiguous CUDA tensors) as src/cuSZp_utility.cu:891-950 computes it, on
+    the device. Synchronises the stream."""
+    if ori.dtype != dec.dtype or ori.numel() != dec.numel() or not (ori.is_contiguous() and dec.is_contiguous()):
+        raise ValueError("ori and dec must be contiguous tensors of one dtype and size")
+    if ori.numel() == 0:
+        raise ValueError("empty tensors")
+    out = (ctypes.c_double * 9)()
+    rc = lib.cuSZp_quality(_ptr(ori), _ptr(dec), ori.numel(), _type_of(ori), out, _stream(stream))
+    if rc:
+        raise CuSZpError(rc, "quality")
+    return dict(zip(QUALITY_FIELDS, list(out)))
+
+
 def compress_tensor_rel(t, rel_eb: float, mode="plain", dim: int = 1, dims: Optional[Tuple[int, int, int]] = None,
                         stream=None):
     """compress_tensor with a relative bound: returns (uint8 tensor, size, absolute eb)."""

@@ -276,6 +276,17 @@ int cuSZp_value_range(const void* d_data, size_t nbEle, cuszp_type_t type, doubl
 int cuSZp_rel_error_bound(const void* d_data, size_t nbEle, cuszp_type_t type, float rel, float* errorBound,
                           hipStream_t stream);
 
+/* Reconstruction quality of d_decData against d_oriData (nbEle values of
+ * `type`), the figures of the reference's computePSNR
+ * (src/cuSZp_utility.cu:891-950 of the reference) computed on the device:
+ * metrics[0] mse, [1] value range of the original, [2] PSNR (dB), [3] L2 norm
+ * of the error, [4] that norm over the L2 norm of d_decData, [5] NRMSE,
+ * [6] max |error|, [7] max |error| / |original| over nonzero originals,
+ * [8] correlation coefficient. f32 data uses the reference's float per-element
+ * arithmetic with double sums. Synchronises `stream`. */
+int cuSZp_quality(const void* d_oriData, const void* d_decData, size_t nbEle, cuszp_type_t type, double* metrics,
+                  hipStream_t stream);
+
 /* Library version string ("cuszp_amd <semver> gfx950"). */
 const char* cuSZp_version(void);
 

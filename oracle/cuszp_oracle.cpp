@@ -25,6 +25,7 @@
 //   block counts ............ src/cuSZp_entry_2D_f32.cu:21-23, src/cuSZp_entry_3D_f32.cu:21-24
 //   decoded value ........... src/cuSZp_kernels_1D_f32.cu:490 (currQuant * eb * 2)
 //   relative bound .......... examples/cuSZp.cpp:176-210 (value range loop, float product)
+//   quality metrics ......... src/cuSZp_utility.cu:891-950 (computePSNR)
 //
 // Decided semantics where the reference is ambiguous or buggy (each pinned by a
 // test in tests/):
@@ -36,6 +37,7 @@
 //   * each block's payload holds exactly the fields the reference decoder parses
 //     from its header byte (only differs from the reference compressor when a
 //     2D/3D outlier-mode residual equals -2^31, where the reference is broken).
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <cstddef>
@@ -337,6 +339,51 @@ bool geo_ok(int dim, size_t n, uint32_t X, uint32_t Y, uint32_t Z) {
     return n == (size_t)X * Y * Z;
 }
 
+// Reconstruction quality as src/cuSZp_utility.cu:891-950 computes it, one
+// element at a time in the reference's order (f32: float error, squares and
+// relative error, double sums; f64: double throughout). out: mse, range, psnr,
+// normErr, normErr_norm, nrmse, max |err|, max pointwise relative error,
+// correlation (acEff).
+template <class T>
+void quality_t(const T* ori, const T* dec, size_t n, double* out) {
+    double mx = ori[0], mn = ori[0];
+    double diffMax = dec[0] > ori[0] ? dec[0] - ori[0] : ori[0] - dec[0];
+    double sum1 = 0, sum2 = 0, sum22 = 0;
+    for (size_t i = 0; i < n; i++) {
+        sum1 += ori[i];
+        sum2 += dec[i];
+        sum22 += (T)(dec[i] * dec[i]);
+    }
+    const double mean1 = sum1 / n, mean2 = sum2 / n;
+    double sum3 = 0, sum4 = 0, sum = 0, prodSum = 0, maxrel = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (mx < ori[i]) mx = ori[i];
+        if (mn > ori[i]) mn = ori[i];
+        const T d = dec[i] - ori[i];
+        const T err = d < 0 ? -d : d;
+        if (ori[i] != 0) {
+            const T a = ori[i] < 0 ? -ori[i] : ori[i];
+            const double rel = (T)(err / a);
+            if (maxrel < rel) maxrel = rel;
+        }
+        if (diffMax < err) diffMax = err;
+        prodSum += (ori[i] - mean1) * (dec[i] - mean2);
+        sum3 += (ori[i] - mean1) * (ori[i] - mean1);
+        sum4 += (dec[i] - mean2) * (dec[i] - mean2);
+        sum += (T)(err * err);
+    }
+    const double mse = sum / n, range = mx - mn;
+    out[0] = mse;
+    out[1] = range;
+    out[2] = 20 * std::log10(range) - 10 * std::log10(mse);
+    out[3] = std::sqrt(sum);
+    out[4] = std::sqrt(sum) / std::sqrt(sum22);
+    out[5] = std::sqrt(mse) / range;
+    out[6] = diffMax;
+    out[7] = maxrel;
+    out[8] = (prodSum / n) / std::sqrt(sum3 / n) / std::sqrt(sum4 / n);
+}
+
 }  // namespace
 
 extern "C" {
@@ -415,6 +462,12 @@ float cuszp_oracle_rel_eb(const void* data, int type, size_t n, float rel, doubl
     *mn = lo;
     *mx = hi;
     return rel * (float)(hi - lo);
+}
+
+void cuszp_oracle_quality(const void* ori, const void* dec, int type, size_t n, double* out) {
+    if (!n) return;
+    if (type == TYPE_F32) quality_t<float>((const float*)ori, (const float*)dec, n, out);
+    else quality_t<double>((const double*)ori, (const double*)dec, n, out);
 }
 
 int cuszp_oracle_max_threads(void) {

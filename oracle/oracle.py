@@ -52,6 +52,8 @@ def lib():
         L.cuszp_oracle_decompress.restype = i32
         L.cuszp_oracle_rel_eb.argtypes = [vp, i32, sz, ctypes.c_float, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
         L.cuszp_oracle_rel_eb.restype = ctypes.c_float
+        L.cuszp_oracle_quality.argtypes = [vp, vp, i32, sz, ctypes.POINTER(dbl)]
+        L.cuszp_oracle_quality.restype = None
         L.cuszp_oracle_max_threads.argtypes = []
         L.cuszp_oracle_max_threads.restype = i32
         _lib = L
@@ -91,6 +93,19 @@ def rel_eb(data: np.ndarray, rel: float):
     mn, mx = ctypes.c_double(0), ctypes.c_double(0)
     eb = lib().cuszp_oracle_rel_eb(data.ctypes.data, typ, data.size, rel, ctypes.byref(mn), ctypes.byref(mx))
     return float(eb), mn.value, mx.value
+
+
+QUALITY_FIELDS = ("mse", "range", "psnr", "norm_err", "norm_err_norm", "nrmse", "max_abs_err", "max_rel_err",
+                  "correlation")
+
+
+def quality(ori: np.ndarray, dec: np.ndarray) -> dict:
+    """Reconstruction quality as src/cuSZp_utility.cu:891-950 computes it (sequential)."""
+    ori = np.ascontiguousarray(ori)
+    dec = np.ascontiguousarray(dec, dtype=ori.dtype)
+    out = (ctypes.c_double * 9)()
+    lib().cuszp_oracle_quality(ori.ctypes.data, dec.ctypes.data, 0 if ori.dtype == np.float32 else 1, ori.size, out)
+    return dict(zip(QUALITY_FIELDS, list(out)))
 
 
 def max_threads():

@@ -36,3 +36,28 @@ def test_rel_eb_special_values(oracle):
     x = np.array([0.0, 16777216.0, -1.0], np.float32)
     eb, mn, mx = oracle.rel_eb(x, 1.0)
     assert eb == float(np.float32(16777216.0) - np.float32(-1.0)) == 16777216.0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_quality_matches_numpy(oracle, dtype):
+    """computePSNR (src/cuSZp_utility.cu:891-950) against the textbook formulas."""
+    rng = np.random.default_rng(7)
+    x = (rng.standard_normal(200_001) * 3).astype(dtype)
+    x[::97] = 0
+    y = (x + rng.uniform(-1e-2, 1e-2, x.size)).astype(dtype)
+    q = oracle.quality(x, y)
+    xd, yd = x.astype(np.float64), y.astype(np.float64)
+    err = np.abs(yd - xd)
+    mse = np.mean(err ** 2)
+    rng_ = xd.max() - xd.min()
+    assert q["range"] == rng_
+    assert q["max_abs_err"] == err.max()
+    nz = x != 0
+    assert q["max_rel_err"] == pytest.approx(np.max(err[nz] / np.abs(xd[nz])), rel=1e-6)
+    assert q["mse"] == pytest.approx(mse, rel=1e-6)
+    assert q["psnr"] == pytest.approx(20 * np.log10(rng_) - 10 * np.log10(mse), rel=1e-6)
+    assert q["norm_err"] == pytest.approx(np.sqrt(np.sum(err ** 2)), rel=1e-6)
+    assert q["norm_err_norm"] == pytest.approx(np.sqrt(np.sum(err ** 2)) / np.sqrt(np.sum(yd ** 2)), rel=1e-6)
+    assert q["nrmse"] == pytest.approx(np.sqrt(mse) / rng_, rel=1e-6)
+    assert q["correlation"] == pytest.approx(np.corrcoef(xd, yd)[0, 1], rel=1e-9)
+    assert oracle.quality(x, x)["max_abs_err"] == 0.0
