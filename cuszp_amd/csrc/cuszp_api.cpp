@@ -124,7 +124,13 @@ TilePlan plan_tiles(int dim, int type, bool compress, const Geom& g) {
     }();
     const uint64_t groups = (dim == CUSZP_DIM_1D) ? g.H / 2 : g.nblocks;   // 64-element units
     int waves = forced;
-    if (!waves) waves = 4;
+    if (!waves) {
+        // 8-wave tiles halve the lookbacks per byte (measured faster from 16 Mi
+        // values up); arrays of fewer than 128 such tiles (16 MiB) keep 4-wave
+        // tiles, which spread over twice as many workgroups
+        const uint64_t per8 = tile_groups(type == CUSZP_TYPE_DOUBLE, 8);
+        waves = (groups + per8 - 1) / per8 >= 128 ? 8 : 4;
+    }
     (void)compress;   // both directions use the same tile shape
     const uint64_t per_tile = tile_groups(type == CUSZP_TYPE_DOUBLE, waves);
     return TilePlan{waves, (groups + per_tile - 1) / per_tile};

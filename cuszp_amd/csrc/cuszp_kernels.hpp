@@ -790,7 +790,6 @@ struct SmemD {
     uint64_t wtot[WAVES];             // per-wave byte counts
     uint64_t excl;                    // the tile's exclusive prefix (wave 0's lookback)
     Meta meta[2][WAVES][32];          // per-block metadata of the chunk in work and the next one
-    uint32_t aux[2][WAVES][32];       // outlier |d| of element 0 per block, same two chunks
     uint8_t hdr[WAVES][128];          // header bytes of the wave's blocks
 };
 
@@ -1238,10 +1237,11 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     const bool fast = groups_full<DIM, kChunks * G>(geo, wgrp0) && (DIM == 1 || geo.Bx % G == 0);
 
     // Bookkeeping of chunk c (payload offset `coff` within the wave): metadata
-    // into slot s, the outlier |d0| of selected blocks into aux[s]; returns the
-    // chunk's byte count. Loads that are not needed read the stream's first
-    // word instead (always inside the caller's buffer: H >= 4), so the code
-    // has no branches around memory operations.
+    // into slot s; for outlier streams lane j also loads block j's |d0| prefix
+    // word into PR[s] (masked by PK[s] at the use, so nothing waits for the
+    // load here). Returns the chunk's byte count. Loads that are not needed read
+    // the zero word, so the code has no branches around memory operations.
+    uint32_t PR0 = 0, PR1 = 0, PK0 = 0, PK1 = 0;
     auto bookkeep = [&](int c, uint32_t coff, int s) -> uint32_t {
         const bool blk = lane < NB;
         const uint32_t byte = blk ? (uint32_t)sm.hdr[wave][c * NB + lane] : 0u;
@@ -1257,9 +1257,15 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
             // a selected block starts with nb <= 4 prefix bytes and then a 4-byte
             // sign word, so one u32 read at the block start stays inside the block
             const bool pre = ok && v.sel && intact;
-            uint32_t o = load_u32u(pre ? wbase + off : cmp);
-            o = pre ? (v.nb < 4 ? o & ((1u << (8 * v.nb)) - 1) : o) : 0u;
-            if (blk) sm.aux[s][wave][lane] = o;
+            const uint32_t o = load_u32u(pre ? wbase + off : zw);
+            const uint32_t k = pre ? (v.nb < 4 ? (1u << (8 * v.nb)) - 1 : ~0u) : 0u;
+            if (s == 0) {
+                PR0 = o;
+                PK0 = k;
+            } else {
+                PR1 = o;
+                PK1 = k;
+            }
         }
         return rdlane(incl, kWaveSize - 1);
     };
@@ -1286,7 +1292,10 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
             }
         }
         if constexpr (MODE == kOutlier) {   // element 0 of a selected block (0 when not intact)
-            const uint32_t e0 = sm.aux[s][wave][BPG * g + (DIM == 1 ? h : 0)];
+            const uint32_t pv = s == 0 ? (PR0 & PK0) : (PR1 & PK1);   // lane j: block j's |d0|
+            uint32_t e0;
+            if constexpr (DIM == 1) e0 = h ? rdlane(pv, 2 * g + 1) : rdlane(pv, 2 * g);
+            else e0 = rdlane(pv, g);
             a = (m.sel() && l == 7 && (DIM == 1 || h == 0)) ? e0 : a;
         }
         const uint32_t dm = neg ? 0u - a : a;
