@@ -185,6 +185,33 @@ __device__ __forceinline__ uint32_t xpose32(uint32_t x, const XposeConsts& c) {
     x = bfi(c.mk1, rotr(swizzle_xor<1>(x), c.sh1), x);
     return x;
 }
+// The same transpose on N independent values, stage by stage, so the N
+// swizzles of a stage are in flight together (one LDS round trip per stage
+// instead of one per value).
+template <int N>
+__device__ __forceinline__ void xpose32_n(uint32_t* x, const XposeConsts& c) {
+    uint32_t t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = swizzle_xor<16>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = __builtin_amdgcn_perm(t[i], x[i], c.sel16);
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = swizzle_xor<8>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = __builtin_amdgcn_perm(t[i], x[i], c.sel8);
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = swizzle_xor<4>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk4, rotr(t[i], c.sh4), x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = swizzle_xor<2>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk2, rotr(t[i], c.sh2), x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) t[i] = swizzle_xor<1>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk1, rotr(t[i], c.sh1), x[i]);
+}
 
 // Inclusive prefix sums (wrapping u32).
 __device__ __forceinline__ uint32_t scan_row16(uint32_t v) {
@@ -772,6 +799,12 @@ constexpr int kCompressWavesPerSimd = 6;
 // Waves per SIMD the decompress kernels are built for: 8 needs at most 64
 // VGPRs and few enough SGPRs for eight waves' scalar files.
 constexpr int kDecompressWavesPerSimd = 8;
+// Raw buffer resources (gfx9): word 3 of the descriptor, and an offset that is
+// always past a resource's range (the range of one wave's payload is < 2^31).
+constexpr int kBufferWord3 = 0x00020000;
+constexpr uint32_t kAbsent = 0x80000000u;
+// Groups whose bit transposes the decompress loop runs side by side.
+constexpr int kXposeBatch = 4;
 
 // Shared memory of a compress workgroup.
 template <int WAVES>
@@ -1182,9 +1215,6 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     const uint32_t l = (uint32_t)lane & 31, h = (uint32_t)lane >> 5;
     const Geom& geo = cur.geo;
     const uint8_t* __restrict__ cmp = (const uint8_t*)cur.src;
-    // a word that is always 0: the guard words in front of the status array are
-    // never written (lanes without a stream word load it, so nothing needs zeroing)
-    const uint8_t* const zw = (const uint8_t*)(sc.status - 1);
     const uint64_t lt = tile - cur.tile0;
     const uint64_t wgrp0 = (lt * WAVES + (uint64_t)wave) * (kChunks * G);   // this wave's first group
     const uint64_t wb0 = (DIM == 1) ? 2 * wgrp0 : wgrp0;                     // and first block
@@ -1222,6 +1252,13 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     const uint8_t* const wbase = cmp + uniform64(wave_base);
     const bool intact = wave_base + sm.wtot[wave] <= cur.limit;
     if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
+    // The wave's stream words go through a buffer resource over its payload:
+    // 32-bit offsets, and an offset past the range (kAbsent) reads 0, so lanes
+    // without a word need no address select and no masking; a truncated stream
+    // gets an empty range (every word reads 0).
+    const uint64_t span = cur.limit - wave_base;
+    const int nrec = __builtin_amdgcn_readfirstlane(intact ? (int)(span < 0x7fffffffull ? span : 0x7fffffffull) : 0);
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)wbase, 0, nrec, kBufferWord3);
 
     XposeConsts xc;
     xc.init(lane);
@@ -1256,9 +1293,9 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         if constexpr (MODE == kOutlier) {
             // a selected block starts with nb <= 4 prefix bytes and then a 4-byte
             // sign word, so one u32 read at the block start stays inside the block
-            const bool pre = ok && v.sel && intact;
-            const uint32_t o = load_u32u(pre ? wbase + off : zw);
-            const uint32_t k = pre ? (v.nb < 4 ? (1u << (8 * v.nb)) - 1 : ~0u) : 0u;
+            const bool pre = ok && v.sel;
+            const uint32_t o = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, pre ? off : kAbsent, 0, 0);
+            const uint32_t k = v.nb < 4 ? (1u << (8 * v.nb)) - 1 : ~0u;
             if (s == 0) {
                 PR0 = o;
                 PK0 = k;
@@ -1269,20 +1306,18 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         }
         return rdlane(incl, kWaveSize - 1);
     };
-    // stream word of this lane in group g of the chunk in slot s: a lane without
-    // a word (or any lane of a truncated stream) loads the zero word, so there
-    // is no branch around the load and nothing to mask at the use
+    // stream word of this lane in group g of the chunk in slot s (0 for a lane
+    // without a word and for every lane of a truncated stream)
     auto word = [&](int s, int g) -> uint32_t {
         const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
         const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
-        const bool has = intact && m.has(l);
-        return load_u32u(has ? wbase + (m.off() + (wideg ? wo_wide : wo)) : zw);
+        const uint32_t o = m.off() + (wideg ? wo_wide : wo);
+        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, m.has(l) ? o : kAbsent, 0, 0);
     };
-    // decodes group g of the chunk in slot s from its loaded word w
-    auto decode = [&](int s, int g, uint32_t w, auto widec) -> T {
+    // decodes group g of the chunk in slot s from its word after the transpose
+    auto decode_x = [&](int s, int g, uint32_t x, auto widec) -> T {
         constexpr bool WIDE = decltype(widec)::value;   // 2D/3D 32-plane blocks possible
         const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
-        const uint32_t x = xpose32(w, xc);
         uint32_t a = x & 0x7fffffffu, neg = x >> 31;   // lane l: sign in bit 31 over the magnitude bits
         if constexpr (WIDE) {
             if (m.wide()) {   // 32 planes: the sign words sit in their own two words
@@ -1306,6 +1341,8 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         // a block without payload (or of a truncated stream) decoded all-zero words: q = 0
         return (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
     };
+    // decodes group g of the chunk in slot s from its loaded word w
+    auto decode = [&](int s, int g, uint32_t w, auto widec) -> T { return decode_x(s, g, xpose32(w, xc), widec); };
 
     // 2D/3D blocks of 32 planes (not in outlier streams) take the general loop
     bool wide_blocks = false;
