@@ -803,7 +803,7 @@ constexpr int kDecompressWavesPerSimd = 8;
 // always past a resource's range (the range of one wave's payload is < 2^31).
 constexpr int kBufferWord3 = 0x00020000;
 constexpr uint32_t kAbsent = 0x80000000u;
-// Groups whose bit transposes the decompress loop runs side by side.
+// Groups whose bit transposes run side by side (a divisor of the groups per chunk).
 constexpr int kXposeBatch = 4;
 
 // Shared memory of a compress workgroup.
@@ -962,6 +962,64 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                 constexpr bool WIDE = decltype(widec)::value;
                 bk1 = bk2 = bko = big = 0;
                 uint64_t anywide = 0;
+                if constexpr (!WIDE) {
+                    // the words of all groups first, then their transposes side by
+                    // side (one LDS round trip per stage for kXposeBatch groups),
+                    // then the plane masks
+                    static_for<0, G>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array gives q = 0
+                        roll_fn(g);
+                        uint32_t d;
+                        if constexpr (MODE == kFixed) {
+                            d = q;
+                        } else if constexpr (DIM == 1) {
+                            uint32_t qp = dpp<kDppWaveShr1>(q);
+                            qp = (lane & 31) ? qp : 0u;
+                            d = q - qp;
+                        } else {
+                            uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
+                            qp = (pl >= 0) ? qp : 0u;
+                            d = q - qp;
+                        }
+                        if constexpr (MODE != kFixed) d = ((vm >> g) & 1) ? d : 0u;   // residuals of padding are 0
+                        const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
+                        const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
+                        if constexpr (DIM != 1) anywide |= ballot((int32_t)a < 0);
+                        // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
+                        // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
+                        W[g] = bfi(0x80000000u, dm, a);
+                        if constexpr (MODE == kOutlier) {
+                            if constexpr (DIM == 1) {
+                                bko = wrlane<2 * g>(rdlane(a, 7), bko);
+                                bko = wrlane<2 * g + 1>(rdlane(a, 39), bko);
+                            } else {
+                                bko = wrlane<g>(rdlane(a, 7), bko);
+                            }
+                        }
+                    });
+                    constexpr int XB = kXposeBatch < G ? kXposeBatch : G;
+#pragma unroll
+                    for (int g = 0; g < G; g += XB) xpose32_n<XB>(W + g, xc);
+                    static_for<0, G>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        const uint64_t m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
+                        uint64_t m2 = 0;
+                        if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
+                        if constexpr (DIM == 1) {
+                            bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
+                            bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
+                            if constexpr (MODE == kOutlier) {
+                                bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
+                                bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
+                            }
+                        } else {
+                            bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
+                            if constexpr (MODE == kOutlier) bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
+                        }
+                    });
+                    return anywide;
+                }
                 static_for<0, G>([&](auto gc) {
                     constexpr int g = decltype(gc)::value;
                     const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array gives q = 0
@@ -1367,8 +1425,14 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
             T* orun;
             if constexpr (DIM == 1) orun = out + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
             else orun = out + block_origin<DIM>(geo, wkc) + lg.off;
+            uint32_t X[G];
 #pragma unroll
-            for (int g = 0; g < G; ++g) orun[step * g] = decode(s, g, W[g], std::false_type{});
+            for (int g = 0; g < G; ++g) X[g] = W[g];
+            constexpr int XB = kXposeBatch < G ? kXposeBatch : G;
+#pragma unroll
+            for (int g = 0; g < G; g += XB) xpose32_n<XB>(X + g, xc);
+#pragma unroll
+            for (int g = 0; g < G; ++g) orun[step * g] = decode_x(s, g, X[g], std::false_type{});
             if constexpr (DIM != 1) wkc.step(geo, G);
         };
         static_assert(kChunks % 2 == 0, "chunks go in pairs");
