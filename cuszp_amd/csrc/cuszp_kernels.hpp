@@ -805,6 +805,11 @@ constexpr int kBufferWord3 = 0x00020000;
 constexpr uint32_t kAbsent = 0x80000000u;
 // Groups whose bit transposes run side by side (a divisor of the groups per chunk).
 constexpr int kXposeBatch = 4;
+// 2D/3D decompress: a chunk's G blocks along x are rows of 128 (3D) or 256 (2D)
+// bytes; they pass through LDS, each row padded by 16 bytes (fewer bank
+// conflicts), so the global stores write whole rows 16 bytes per lane.
+constexpr int kOutPad = 16;
+constexpr int kOutStage = 16 * (128 + kOutPad);   // the larger of 16 x 128 B and 8 x 256 B, padded
 
 // Shared memory of a compress workgroup.
 template <int WAVES>
@@ -824,6 +829,7 @@ struct SmemD {
     uint64_t excl;                    // the tile's exclusive prefix (wave 0's lookback)
     Meta meta[2][WAVES][32];          // per-block metadata of the chunk in work and the next one
     uint8_t hdr[WAVES][128];          // header bytes of the wave's blocks
+    uint4 ostage[WAVES][kOutStage / 16];   // 2D/3D: a chunk's decoded rows, padded
 };
 
 // Load of one element per lane. 1D loads cover whole cache lines and stream
@@ -1421,19 +1427,67 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         // would also wait for that store).
         Walker wkc;   // 2D/3D: the chunk being decoded
         if constexpr (DIM != 1) wkc.init(wgrp0, geo);
+        // 2D/3D output staging: block width BW, rows of RL values, row stride RS
+        // (values) in LDS; lane e of group g goes to row ROW, column g * BW + COL.
+        // Read back, lane j (and j + 64) takes 16 bytes: row j / VPR, vector j % VPR.
+        constexpr int BW = (DIM == 3) ? 4 : 8;
+        constexpr int RL = G * BW;
+        constexpr int RS = RL + kOutPad / (int)sizeof(T);
+        constexpr int VPR = RL * (int)sizeof(T) / 16;          // 16-byte vectors per row
+        constexpr int NR = 64 / BW;                            // rows: 16 (3D) or 8 (2D)
+        static_assert(NR * RS * (int)sizeof(T) <= kOutStage, "output staging too small");
+        static_assert(2 * 64 * 16 == NR * RL * (int)sizeof(T), "two 16-byte vectors per lane");
+        uint32_t st_w = 0, st_r = 0;   // LDS byte offsets: this lane's value / first vector
+        uint64_t roff = 0, rdelta = 0;  // global element offsets of the rows read back
+        if constexpr (DIM != 1) {
+            uint32_t rw, cw;
+            if constexpr (DIM == 3) {
+                rw = lg.zz * 4 + lg.yy;
+                cw = lg.xx;
+            } else {
+                rw = lg.r;
+                cw = lg.c;
+            }
+            st_w = (rw * RS + cw) * (uint32_t)sizeof(T);
+            const uint32_t rr = (uint32_t)lane / VPR, cv = (uint32_t)lane % VPR;
+            st_r = rr * RS * (uint32_t)sizeof(T) + cv * 16;
+            // the second vector is 64 / VPR rows further down
+            if constexpr (DIM == 3) {
+                roff = (uint64_t)(rr >> 2) * geo.XY + (uint64_t)(rr & 3) * geo.X + cv * (16 / sizeof(T));
+                rdelta = (uint64_t)(64 / VPR / 4) * geo.XY;
+            } else {
+                roff = (uint64_t)rr * geo.X + cv * (16 / sizeof(T));
+                rdelta = (uint64_t)(64 / VPR) * geo.X;
+            }
+        }
         auto decode_chunk = [&](int c, int s, const uint32_t (&W)[G]) {
-            T* orun;
-            if constexpr (DIM == 1) orun = out + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
-            else orun = out + block_origin<DIM>(geo, wkc) + lg.off;
             uint32_t X[G];
 #pragma unroll
             for (int g = 0; g < G; ++g) X[g] = W[g];
             constexpr int XB = kXposeBatch < G ? kXposeBatch : G;
 #pragma unroll
             for (int g = 0; g < G; g += XB) xpose32_n<XB>(X + g, xc);
+            if constexpr (DIM == 1) {
+                T* const orun = out + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
 #pragma unroll
-            for (int g = 0; g < G; ++g) orun[step * g] = decode_x(s, g, X[g], std::false_type{});
-            if constexpr (DIM != 1) wkc.step(geo, G);
+                for (int g = 0; g < G; ++g) orun[step * g] = decode_x(s, g, X[g], std::false_type{});
+            } else {
+                lds_u8* const stg = (lds_u8*)sm.ostage[wave];
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                    *(__attribute__((address_space(3))) T*)(stg + st_w + g * BW * sizeof(T)) =
+                        decode_x(s, g, X[g], std::false_type{});
+                __builtin_amdgcn_wave_barrier();
+                typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+                typedef uint32_t v4uu __attribute__((ext_vector_type(4), aligned(1)));
+                const v4u v0 = *(__attribute__((address_space(3))) v4u*)(stg + st_r);
+                const v4u v1 = *(__attribute__((address_space(3))) v4u*)(stg + st_r + (64 / VPR) * RS * sizeof(T));
+                __builtin_amdgcn_wave_barrier();
+                T* const ob = out + block_origin<DIM>(geo, wkc);
+                *(v4uu*)(ob + roff) = v0;
+                *(v4uu*)(ob + roff + rdelta) = v1;
+                wkc.step(geo, G);
+            }
         };
         static_assert(kChunks % 2 == 0, "chunks go in pairs");
         uint32_t WA[G], WB[G];
