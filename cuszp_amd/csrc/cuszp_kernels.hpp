@@ -1183,15 +1183,13 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             const T* p = chunk_ptr(0);
 #pragma unroll
             for (int g = 0; g < G; ++g) xv[g] = load_elem<DIM>(p + step * g);
-            for (int c = 0; c < kChunks; ++c) {
-                // the next chunk (the last chunk re-reads itself from cache)
-                const bool more = c + 1 < kChunks;
-                if constexpr (DIM != 1) {
-                    if (more) wkn.step(geo, G);
-                }
-                const T* const pn = chunk_ptr(more ? c + 1 : c);
+            for (int c = 0; c + 1 < kChunks; ++c) {
+                // group g of the next chunk loads as soon as group g of this one is coded
+                if constexpr (DIM != 1) wkn.step(geo, G);
+                const T* const pn = chunk_ptr(c + 1);
                 process(c, xv, ~0ull, [&](int g) { xv[g] = load_elem<DIM>(pn + step * g); });
             }
+            process(kChunks - 1, xv, ~0ull, [](int) {});   // the last chunk loads nothing more
         } else {
             // groups holding elements of the array; a chunk past them is all
             // zero: rate bytes 0 in every mode and no payload, so it only writes
@@ -1494,19 +1492,25 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         uint32_t coff = bookkeep(0, 0, 0);
 #pragma unroll
         for (int g = 0; g < G; ++g) WA[g] = word(0, g);
-        for (int c = 0; c < kChunks; c += 2) {
+        for (int c = 0; c + 2 < kChunks; c += 2) {
             coff += bookkeep(c + 1, coff, 1);
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int g = 0; g < G; ++g) WB[g] = word(1, g);
             decode_chunk(c, 0, WA);
-            // the chunk after next (past the last one: chunk c + 1 again, unused)
-            coff += bookkeep(c + 2 < kChunks ? c + 2 : c + 1, coff, 0);
+            coff += bookkeep(c + 2, coff, 0);
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int g = 0; g < G; ++g) WA[g] = word(0, g);
             decode_chunk(c + 1, 1, WB);
         }
+        // the last pair: nothing more to load after it
+        coff += bookkeep(kChunks - 1, coff, 1);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int g = 0; g < G; ++g) WB[g] = word(1, g);
+        decode_chunk(kChunks - 2, 0, WA);
+        decode_chunk(kChunks - 1, 1, WB);
     } else {
         // ---- general: per-element checks, 32-plane blocks -------------------------
         Walker wk;
