@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcuszp_amd.so")
+CLI = os.path.join(HERE, "cuszp")
 SOURCES = ["kern_batch_c.hip", "kern_batch_d.hip", "kern_f32.hip", "kern_f64.hip", "kern_range.hip", "kern_quality.hip",
            "cuszp_api.cpp",
            "cuszp_batch.cpp", "cuszp_serve.cpp"]
@@ -57,6 +58,11 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # the command-line front end (cuszp_amd/cuszp), linked against the library
+    tmp = CLI + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-x", "hip", os.path.join(HERE, "cli", "cuszp_cli.cpp"),
+          "-x", "none", "-L" + HERE, "-lcuszp_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, CLI)
     return LIB
 
 
