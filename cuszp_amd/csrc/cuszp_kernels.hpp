@@ -621,6 +621,27 @@ __device__ __forceinline__ uint32_t inverse_lorenzo(uint32_t d, int e) {
     }
 }
 
+// The same sums for N independent groups, step by step, so each DPP step of
+// one group fills the other groups' hazard slots instead of s_nop.
+template <int DIM, int N>
+__device__ __forceinline__ void inverse_lorenzo_n(uint32_t* d, int e) {
+    if constexpr (DIM == 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] += dpp<kDppRowShr1>(d[i]);
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] += dpp<kDppRowShr2>(d[i]);
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] += dpp<kDppRowShr4>(d[i]);
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] += dpp<kDppRowShr8>(d[i]);
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] += dpp<kDppBcast15, 0xa>(d[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = inverse_lorenzo<DIM>(d[i], e);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // per-block metadata handed from the bookkeeping lanes to the stream-word
 // lanes through LDS (one 8-byte entry per block):
@@ -1377,7 +1398,9 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, m.has(l) ? o : kAbsent, 0, 0);
     };
     // decodes group g of the chunk in slot s from its word after the transpose
-    auto decode_x = [&](int s, int g, uint32_t x, auto widec) -> T {
+    // residuals (natural element order) of group g of the chunk in slot s from
+    // its word after the transpose
+    auto residual_x = [&](int s, int g, uint32_t x, auto widec) -> uint32_t {
         constexpr bool WIDE = decltype(widec)::value;   // 2D/3D 32-plane blocks possible
         const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
         uint32_t a = x & 0x7fffffffu, neg = x >> 31;   // lane l: sign in bit 31 over the magnitude bits
@@ -1396,12 +1419,15 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
             a = (m.sel() && l == 7 && (DIM == 1 || h == 0)) ? e0 : a;
         }
         const uint32_t dm = neg ? 0u - a : a;
-        const uint32_t d = dpp<kDppHalfMirror>(dm);   // back to natural element order
-        uint32_t q;
-        if constexpr (MODE == kFixed) q = d;
-        else q = inverse_lorenzo<DIM>(d, lane);
-        // a block without payload (or of a truncated stream) decoded all-zero words: q = 0
-        return (T)(int32_t)q * eb * T(2);   // src/cuSZp_kernels_1D_f32.cu:490
+        return dpp<kDppHalfMirror>(dm);   // back to natural element order
+    };
+    // the value of code q (a block without payload, or of a truncated stream,
+    // decoded all-zero words: q = 0)
+    auto value_of = [&](uint32_t q) -> T { return (T)(int32_t)q * eb * T(2); };   // src/cuSZp_kernels_1D_f32.cu:490
+    auto decode_x = [&](int s, int g, uint32_t x, auto widec) -> T {
+        const uint32_t d = residual_x(s, g, x, widec);
+        if constexpr (MODE == kFixed) return value_of(d);
+        else return value_of(inverse_lorenzo<DIM>(d, lane));
     };
     // decodes group g of the chunk in slot s from its loaded word w
     auto decode = [&](int s, int g, uint32_t w, auto widec) -> T { return decode_x(s, g, xpose32(w, xc), widec); };
@@ -1465,16 +1491,22 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
             constexpr int XB = kXposeBatch < G ? kXposeBatch : G;
 #pragma unroll
             for (int g = 0; g < G; g += XB) xpose32_n<XB>(X + g, xc);
+            // X[g]: residuals, then (batch by batch) codes
+#pragma unroll
+            for (int g = 0; g < G; ++g) X[g] = residual_x(s, g, X[g], std::false_type{});
+            if constexpr (MODE != kFixed) {
+#pragma unroll
+                for (int g = 0; g < G; g += XB) inverse_lorenzo_n<DIM, XB>(X + g, lane);
+            }
             if constexpr (DIM == 1) {
                 T* const orun = out + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
 #pragma unroll
-                for (int g = 0; g < G; ++g) orun[step * g] = decode_x(s, g, X[g], std::false_type{});
+                for (int g = 0; g < G; ++g) orun[step * g] = value_of(X[g]);
             } else {
                 lds_u8* const stg = (lds_u8*)sm.ostage[wave];
 #pragma unroll
                 for (int g = 0; g < G; ++g)
-                    *(__attribute__((address_space(3))) T*)(stg + st_w + g * BW * sizeof(T)) =
-                        decode_x(s, g, X[g], std::false_type{});
+                    *(__attribute__((address_space(3))) T*)(stg + st_w + g * BW * sizeof(T)) = value_of(X[g]);
                 __builtin_amdgcn_wave_barrier();
                 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
                 typedef uint32_t v4uu __attribute__((ext_vector_type(4), aligned(1)));
