@@ -429,9 +429,11 @@ int cuSZp_server_replay(cuszp_server_t s, const cuszp_request_t* reqs, size_t co
     if (!s || (count && (!reqs || !latency_s || !statuses))) return CUSZP_ERROR_INVALID_ARGUMENT;
     for (size_t i = 1; i < count; ++i)
         if (reqs[i].arrival_s < reqs[i - 1].arrival_s) return CUSZP_ERROR_INVALID_ARGUMENT;
-    if (sequential) {
+    {
+        // the replay owns the server: no other request may be outstanding (the
+        // sequential form also uses the pool's first bytes as its output)
         std::lock_guard<std::mutex> lk(s->mu);
-        if (!s->held.empty()) return CUSZP_ERROR_INVALID_ARGUMENT;   // the pool must be free
+        if (!s->held.empty() || !s->done.empty()) return CUSZP_ERROR_INVALID_ARGUMENT;
     }
     const double t0 = now_s();
     if (sequential) {

@@ -253,8 +253,8 @@ void cuSZp_server_destroy(cuszp_server_t server);
  * completion is released at once, and latency_s[i] = completion seen -
  * arrival. sequential != 0 instead runs the requests one at a time in arrival
  * order through cuSZp_compress_ex / cuSZp_decompress_ex (output at the start
- * of the pool; the server must hold no requests). *makespan_s = time until the
- * last completion. */
+ * of the pool). The server must hold no other request (CUSZP_ERROR_INVALID_ARGUMENT
+ * otherwise). *makespan_s = time until the last completion. */
 typedef struct {
     int direction;
     cuszp_batch_item_t item;

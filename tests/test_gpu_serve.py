@@ -174,3 +174,23 @@ def test_server_replay(device, sequential):
         assert st == [0] * len(reqs)
         assert all(v > 0 for v in lat) and mk >= 0.0005 * (len(reqs) - 1)
         assert S.stats() == (0, 0, 0)
+
+
+def test_server_replay_needs_an_idle_server(device):
+    import torch
+    from cuszp_amd import pycuSZp as P
+
+    x = torch.from_numpy(mixed_field(10_000, seed=9)).to(device)
+    pool = torch.empty(8 << 20, dtype=torch.uint8, device=device)
+    with P.Server(pool, lanes=2, max_batch=8) as S:
+        rid = S.submit_compress(x, 1e-3, 1)
+        req = [(P.BATCH_COMPRESS, P.batch_item(x, None, x.numel(), 0, 1e-3, 1, (0, 0, 0), 0, 1), 0.0)]
+        for seq in (False, True):
+            with pytest.raises(P.CuSZpError) as e:
+                S.replay(req, sequential=seq)
+            assert e.value.status == 1
+        done = S.poll(wait=True)
+        assert [d[0] for d in done] == [rid]
+        S.release(rid)
+        lat, st, _ = S.replay(req)
+        assert st == [0]
