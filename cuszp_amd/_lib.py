@@ -55,6 +55,7 @@ EXPORTED = (
         "cuSZp_value_range",
         "cuSZp_rel_error_bound",
         "cuSZp_quality",
+        "cuSZp_ssim",
     ]
 )
 
@@ -178,5 +179,7 @@ def load() -> ctypes.CDLL:
     L.cuSZp_rel_error_bound.restype = i32
     L.cuSZp_quality.argtypes = [vp, vp, sz, i32, ctypes.POINTER(f64), vp]
     L.cuSZp_quality.restype = i32
+    L.cuSZp_ssim.argtypes = [vp, vp, Uint3, i32, ctypes.POINTER(f64), vp]
+    L.cuSZp_ssim.restype = i32
     _lib = L
     return L

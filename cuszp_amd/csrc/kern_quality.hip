@@ -1,6 +1,7 @@
 // Reconstruction quality of a decompressed array against its original, on the
 // device: the figures of the reference's computePSNR (src/cuSZp_utility.cu:891-950)
-// plus the two maxima it tracks, for the batched and serving reports.
+// plus the two maxima it tracks, and its windowed SSIM (:761-889), for the
+// batched and serving reports.
 //
 // Per element the arithmetic is the reference's: for f32 data the error
 // |dec - ori|, its square, dec * dec and the pointwise relative error are
@@ -153,6 +154,80 @@ __global__ void __launch_bounds__(kThreads) pass2_kernel(const T* __restrict__ o
         for (int k = 0; k < 3; ++k) res2[k] = s[k];
 }
 
+// SSIM over 7x7x7 windows moved by 2 along each axis (src/cuSZp_utility.cu:761-889):
+// per window, the original's min and max, both means, both variances and the
+// covariance (double sums of the values), then luminance * contrast * structure
+// with c1 = (0.01 R)^2, c2 = (0.03 R)^2, c3 = c2 / 2, R the window's original
+// range (1 when the range is 0). One thread per window; the mean over windows
+// goes through per-workgroup partials and the last workgroup.
+constexpr int kWin = 7, kWinStep = 2;
+
+template <class T>
+__device__ double window_ssim(const T* __restrict__ x, const T* __restrict__ y, uint64_t s1, uint64_t s0,
+                              uint64_t o0, uint64_t o1, uint64_t o2) {
+    const uint64_t b = o0 + s0 * (o1 + s1 * o2);
+    T xmin = x[b], xmax = x[b];
+    double xs = 0, ys = 0;
+    for (int k = 0; k < kWin; ++k)
+        for (int j = 0; j < kWin; ++j) {
+            const uint64_t r = b + s0 * ((uint64_t)j + s1 * (uint64_t)k);
+#pragma unroll
+            for (int i = 0; i < kWin; ++i) {
+                const T a = x[r + i], c = y[r + i];
+                xmin = a < xmin ? a : xmin;
+                xmax = a > xmax ? a : xmax;
+                xs += (double)a;
+                ys += (double)c;
+            }
+        }
+    constexpr double np = kWin * kWin * kWin;
+    const double mx = xs / np, my = ys / np;
+    double vx = 0, vy = 0, vxy = 0;
+    for (int k = 0; k < kWin; ++k)
+        for (int j = 0; j < kWin; ++j) {
+            const uint64_t r = b + s0 * ((uint64_t)j + s1 * (uint64_t)k);
+#pragma unroll
+            for (int i = 0; i < kWin; ++i) {
+                const double a = (double)x[r + i] - mx, c = (double)y[r + i] - my;
+                vx += a * a;
+                vy += c * c;
+                vxy += a * c;
+            }
+        }
+    vx /= np;
+    vy /= np;
+    vxy /= np;
+    const double sx = sqrt(vx), sy = sqrt(vy);
+    const double rr = (double)(T)(xmax - xmin);   // the range in the values' type
+    const double c1 = rr == 0 ? 0.01 * 0.01 : 0.01 * 0.01 * rr * rr;
+    const double c2 = rr == 0 ? 0.03 * 0.03 : 0.03 * 0.03 * rr * rr;
+    const double c3 = c2 / 2;
+    const double lum = (2 * mx * my + c1) / (mx * mx + my * my + c1);
+    const double con = (2 * sx * sy + c2) / (sx * sx + sy * sy + c2);
+    const double str = (vxy + c3) / (sx * sy + c3);
+    return lum * con * str;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kThreads) ssim_kernel(const T* __restrict__ x, const T* __restrict__ y,
+                                                        uint64_t s1, uint64_t s0, uint64_t nw0, uint64_t nw1,
+                                                        uint64_t nw, double* part, unsigned* ticket, double* res) {
+    __shared__ double sm[(kThreads / 64) * 16];
+    __shared__ bool flag;
+    double v[1] = {0};
+    for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < nw; w += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t w0 = w % nw0, w1 = (w / nw0) % nw1, w2 = w / (nw0 * nw1);
+        v[0] += window_ssim(x, y, s1, s0, w0 * kWinStep, w1 * kWinStep, w2 * kWinStep);
+    }
+    block_sum<T>(v, 1, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    if (!last_block(ticket, &flag)) return;
+    v[0] = 0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += kThreads) v[0] += ld(part + b);
+    block_sum<T>(v, 1, sm);
+    if (threadIdx.x == 0) res[0] = v[0] / (double)nw;
+}
+
 struct State {
     double* part = nullptr;   // kP1 * kMaxBlocks
     unsigned* ticket = nullptr;
@@ -222,7 +297,39 @@ int run(const T* ori, const T* dec, size_t n, double* out, hipStream_t st) {
     return CUSZP_SUCCESS;
 }
 
+template <class T>
+int run_ssim(const T* x, const T* y, cuszp_dims_t d, double* out, hipStream_t st) {
+    if (d.x < (unsigned)kWin || d.y < (unsigned)kWin || d.z < (unsigned)kWin) return CUSZP_ERROR_INVALID_DIMS;
+    int dev = 0;
+    Q_HIP(hipGetDevice(&dev));
+    State* s = nullptr;
+    int rc = state_for(dev, st, s);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const uint64_t nw0 = (d.x - kWin) / kWinStep + 1, nw1 = (d.y - kWin) / kWinStep + 1,
+                   nw2 = (d.z - kWin) / kWinStep + 1, nw = nw0 * nw1 * nw2;
+    uint64_t blocks = (nw + kThreads - 1) / kThreads;
+    if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+    hipLaunchKernelGGL(ssim_kernel<T>, dim3((unsigned)blocks), dim3(kThreads), 0, st, x, y, (uint64_t)d.y,
+                       (uint64_t)d.x, nw0, nw1, nw, s->part, s->ticket, s->res);
+    Q_HIP(hipGetLastError());
+    Q_HIP(hipMemcpyAsync(s->h_res, s->res, sizeof(double), hipMemcpyDeviceToHost, st));
+    Q_HIP(hipStreamSynchronize(st));
+    *out = s->h_res[0];
+    return CUSZP_SUCCESS;
+}
+
 }  // namespace cuszp_quality
+
+extern "C" int cuSZp_ssim(const void* d_oriData, const void* d_decData, cuszp_dims_t dims, cuszp_type_t type,
+                          double* ssim, hipStream_t stream) {
+    if (!d_oriData || !d_decData || !ssim) return CUSZP_ERROR_INVALID_ARGUMENT;
+    if (type == CUSZP_TYPE_FLOAT)
+        return cuszp_quality::run_ssim((const float*)d_oriData, (const float*)d_decData, dims, ssim, stream);
+    if (type == CUSZP_TYPE_DOUBLE)
+        return cuszp_quality::run_ssim((const double*)d_oriData, (const double*)d_decData, dims, ssim, stream);
+    return CUSZP_ERROR_INVALID_ARGUMENT;
+}
 
 extern "C" int cuSZp_quality(const void* d_oriData, const void* d_decData, size_t nbEle, cuszp_type_t type,
                              double* metrics, hipStream_t stream) {

@@ -172,6 +172,20 @@ def quality(ori, dec, stream=None) -> dict:
     return dict(zip(QUALITY_FIELDS, list(out)))
 
 
+def ssim(ori, dec, dims, stream=None) -> float:
+    """Mean SSIM over 7x7x7 windows moved by 2 (src/cuSZp_utility.cu:761-889),
+    on the device; dims = (x, y, z), x fastest, each at least 7."""
+    if ori.dtype != dec.dtype or ori.numel() != dec.numel() or not (ori.is_contiguous() and dec.is_contiguous()):
+        raise ValueError("ori and dec must be contiguous tensors of one dtype and size")
+    if ori.numel() != dims[0] * dims[1] * dims[2]:
+        raise ValueError("dims do not match the number of values")
+    out = ctypes.c_double(0)
+    rc = lib.cuSZp_ssim(_ptr(ori), _ptr(dec), Uint3(*dims), _type_of(ori), ctypes.byref(out), _stream(stream))
+    if rc:
+        raise CuSZpError(rc, "ssim")
+    return out.value
+
+
 def compress_tensor_rel(t, rel_eb: float, mode="plain", dim: int = 1, dims: Optional[Tuple[int, int, int]] = None,
                         stream=None):
     """compress_tensor with a relative bound: returns (uint8 tensor, size, absolute eb)."""

@@ -286,6 +286,12 @@ int cuSZp_rel_error_bound(const void* d_data, size_t nbEle, cuszp_type_t type, f
  * arithmetic with double sums. Synchronises `stream`. */
 int cuSZp_quality(const void* d_oriData, const void* d_decData, size_t nbEle, cuszp_type_t type, double* metrics,
                   hipStream_t stream);
+/* Mean SSIM of d_decData against d_oriData over 7x7x7 windows moved by 2 along
+ * each axis, as the reference's computeSSIM (src/cuSZp_utility.cu:761-889 of
+ * the reference) on a dims.x * dims.y * dims.z volume (x fastest). Every
+ * extent must be at least 7 (CUSZP_ERROR_INVALID_DIMS). Synchronises `stream`. */
+int cuSZp_ssim(const void* d_oriData, const void* d_decData, cuszp_dims_t dims, cuszp_type_t type, double* ssim,
+               hipStream_t stream);
 
 /* Library version string ("cuszp_amd <semver> gfx950"). */
 const char* cuSZp_version(void);

@@ -25,7 +25,7 @@
 //   block counts ............ src/cuSZp_entry_2D_f32.cu:21-23, src/cuSZp_entry_3D_f32.cu:21-24
 //   decoded value ........... src/cuSZp_kernels_1D_f32.cu:490 (currQuant * eb * 2)
 //   relative bound .......... examples/cuSZp.cpp:176-210 (value range loop, float product)
-//   quality metrics ......... src/cuSZp_utility.cu:891-950 (computePSNR)
+//   quality metrics ......... src/cuSZp_utility.cu:891-950 (computePSNR), :761-889 (SSIM)
 //
 // Decided semantics where the reference is ambiguous or buggy (each pinned by a
 // test in tests/):
@@ -384,6 +384,54 @@ void quality_t(const T* ori, const T* dec, size_t n, double* out) {
     out[8] = (prodSum / n) / std::sqrt(sum3 / n) / std::sqrt(sum4 / n);
 }
 
+// Mean SSIM over 7x7x7 windows moved by 2 (src/cuSZp_utility.cu:761-889),
+// windows visited z-outer, x-inner as the reference does; per window the
+// original's range in the values' type, double sums, then the three factors.
+template <class T>
+double ssim_t(const T* x, const T* y, size_t X, size_t Y, size_t Z) {
+    const int W = 7, step = 2;
+    double total = 0;
+    long count = 0;
+    for (size_t o2 = 0; o2 + W <= Z; o2 += step)
+        for (size_t o1 = 0; o1 + W <= Y; o1 += step)
+            for (size_t o0 = 0; o0 + W <= X; o0 += step) {
+                const size_t base = o0 + X * (o1 + Y * o2);
+                T lo = x[base], hi = x[base];
+                double sx = 0, sy = 0;
+                for (int k = 0; k < W; k++)
+                    for (int j = 0; j < W; j++)
+                        for (int i = 0; i < W; i++) {
+                            const size_t e = base + i + X * (j + Y * (size_t)k);
+                            if (lo > x[e]) lo = x[e];
+                            if (hi < x[e]) hi = x[e];
+                            sx += x[e];
+                            sy += y[e];
+                        }
+                const double n = W * W * W, mx = sx / n, my = sy / n;
+                double vx = 0, vy = 0, vxy = 0;
+                for (int k = 0; k < W; k++)
+                    for (int j = 0; j < W; j++)
+                        for (int i = 0; i < W; i++) {
+                            const size_t e = base + i + X * (j + Y * (size_t)k);
+                            vx += (x[e] - mx) * (x[e] - mx);
+                            vy += (y[e] - my) * (y[e] - my);
+                            vxy += (x[e] - mx) * (y[e] - my);
+                        }
+                vx /= n;
+                vy /= n;
+                vxy /= n;
+                const double sgx = std::sqrt(vx), sgy = std::sqrt(vy);
+                const T r = hi - lo;
+                const double c1 = r == 0 ? 0.01 * 0.01 : 0.01 * 0.01 * r * r;
+                const double c2 = r == 0 ? 0.03 * 0.03 : 0.03 * 0.03 * r * r;
+                const double c3 = c2 / 2;
+                total += (2 * mx * my + c1) / (mx * mx + my * my + c1) * ((2 * sgx * sgy + c2) / (sgx * sgx + sgy * sgy + c2)) *
+                         ((vxy + c3) / (sgx * sgy + c3));
+                count++;
+            }
+    return total / count;
+}
+
 }  // namespace
 
 extern "C" {
@@ -468,6 +516,12 @@ void cuszp_oracle_quality(const void* ori, const void* dec, int type, size_t n, 
     if (!n) return;
     if (type == TYPE_F32) quality_t<float>((const float*)ori, (const float*)dec, n, out);
     else quality_t<double>((const double*)ori, (const double*)dec, n, out);
+}
+
+double cuszp_oracle_ssim(const void* ori, const void* dec, int type, size_t X, size_t Y, size_t Z) {
+    if (X < 7 || Y < 7 || Z < 7) return -2.0;
+    if (type == TYPE_F32) return ssim_t<float>((const float*)ori, (const float*)dec, X, Y, Z);
+    return ssim_t<double>((const double*)ori, (const double*)dec, X, Y, Z);
 }
 
 int cuszp_oracle_max_threads(void) {

@@ -54,6 +54,8 @@ def lib():
         L.cuszp_oracle_rel_eb.restype = ctypes.c_float
         L.cuszp_oracle_quality.argtypes = [vp, vp, i32, sz, ctypes.POINTER(dbl)]
         L.cuszp_oracle_quality.restype = None
+        L.cuszp_oracle_ssim.argtypes = [vp, vp, i32, sz, sz, sz]
+        L.cuszp_oracle_ssim.restype = dbl
         L.cuszp_oracle_max_threads.argtypes = []
         L.cuszp_oracle_max_threads.restype = i32
         _lib = L
@@ -106,6 +108,14 @@ def quality(ori: np.ndarray, dec: np.ndarray) -> dict:
     out = (ctypes.c_double * 9)()
     lib().cuszp_oracle_quality(ori.ctypes.data, dec.ctypes.data, 0 if ori.dtype == np.float32 else 1, ori.size, out)
     return dict(zip(QUALITY_FIELDS, list(out)))
+
+
+def ssim(ori: np.ndarray, dec: np.ndarray, dims) -> float:
+    """Mean SSIM as src/cuSZp_utility.cu:761-889 computes it; dims = (x, y, z), x fastest."""
+    ori = np.ascontiguousarray(ori)
+    dec = np.ascontiguousarray(dec, dtype=ori.dtype)
+    x, y, z = dims
+    return lib().cuszp_oracle_ssim(ori.ctypes.data, dec.ctypes.data, 0 if ori.dtype == np.float32 else 1, x, y, z)
 
 
 def max_threads():

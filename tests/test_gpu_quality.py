@@ -31,3 +31,22 @@ def test_quality_matches_oracle(device, oracle, dtype, n):
         else:
             assert not np.isfinite(got[k]) or n == 1, k
     assert got["max_abs_err"] <= eb * (1 + 2 ** -20) + 4 * np.finfo(dtype).eps * float(np.abs(x).max())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ssim_matches_oracle(device, oracle, dtype):
+    import torch
+    from cuszp_amd import pycuSZp as P
+    from tests.workloads import grf_volume
+
+    dims = (40, 36, 30)
+    x = grf_volume(30, 36, 40, seed=6, dtype=dtype).ravel()
+    t = torch.from_numpy(x).to(device)
+    cmp, size = P.compress_tensor(t, 0.5, mode="outlier", dim=3, dims=dims)
+    y = P.decompress_tensor(cmp, size, t.numel(), t.dtype, 0.5, mode="outlier", dim=3, dims=dims)
+    got = P.ssim(t, y, dims)
+    want = oracle.ssim(x, y.cpu().numpy(), dims)
+    assert got == pytest.approx(want, rel=1e-9)
+    assert P.ssim(t, t, dims) == pytest.approx(1.0, abs=1e-12)
+    with pytest.raises(P.CuSZpError):
+        P.ssim(t[: 6 * 36 * 30].contiguous(), y[: 6 * 36 * 30].contiguous(), (6, 36, 30))

@@ -61,3 +61,40 @@ def test_quality_matches_numpy(oracle, dtype):
     assert q["nrmse"] == pytest.approx(np.sqrt(mse) / rng_, rel=1e-6)
     assert q["correlation"] == pytest.approx(np.corrcoef(xd, yd)[0, 1], rel=1e-9)
     assert oracle.quality(x, x)["max_abs_err"] == 0.0
+
+
+def _ssim_numpy(x, y, dims):
+    """Windowed SSIM restated with numpy's sliding windows (float64 statistics;
+    the window range in the values' type)."""
+    X, Y, Z = dims
+    a = x.reshape(Z, Y, X)
+    b = y.reshape(Z, Y, X)
+    wa = np.lib.stride_tricks.sliding_window_view(a, (7, 7, 7))[::2, ::2, ::2]
+    wb = np.lib.stride_tricks.sliding_window_view(b, (7, 7, 7))[::2, ::2, ::2]
+    wa64, wb64 = wa.astype(np.float64), wb.astype(np.float64)
+    mx, my = wa64.mean(axis=(3, 4, 5)), wb64.mean(axis=(3, 4, 5))
+    vx = ((wa64 - mx[..., None, None, None]) ** 2).mean(axis=(3, 4, 5))
+    vy = ((wb64 - my[..., None, None, None]) ** 2).mean(axis=(3, 4, 5))
+    vxy = ((wa64 - mx[..., None, None, None]) * (wb64 - my[..., None, None, None])).mean(axis=(3, 4, 5))
+    r = (wa.max(axis=(3, 4, 5)) - wa.min(axis=(3, 4, 5))).astype(np.float64)
+    c1 = np.where(r == 0, 1e-4, 1e-4 * r * r)
+    c2 = np.where(r == 0, 9e-4, 9e-4 * r * r)
+    sx, sy = np.sqrt(vx), np.sqrt(vy)
+    s = (2 * mx * my + c1) / (mx * mx + my * my + c1) * (2 * sx * sy + c2) / (sx * sx + sy * sy + c2) \
+        * (vxy + c2 / 2) / (sx * sy + c2 / 2)
+    return float(s.mean())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ssim_matches_numpy(oracle, dtype):
+    """SSIM (src/cuSZp_utility.cu:761-889): 1 for identical volumes, and the
+    windowed formulas restated with numpy agree."""
+    rng = np.random.default_rng(5)
+    dims = (20, 15, 11)
+    x = np.cumsum(rng.standard_normal(20 * 15 * 11)).astype(dtype)
+    y = (x + rng.uniform(-0.3, 0.3, x.size)).astype(dtype)
+    assert oracle.ssim(x, x, dims) == pytest.approx(1.0, abs=1e-12)
+    assert oracle.ssim(x, y, dims) == pytest.approx(_ssim_numpy(x, y, dims), rel=1e-9)
+    flat = np.full(x.size, 2.5, dtype)   # zero range: the constants fall back to (0.01)^2, (0.03)^2
+    assert oracle.ssim(flat, flat, dims) == pytest.approx(1.0, abs=1e-12)
+    assert oracle.ssim(x[:6 * 15 * 11], y[:6 * 15 * 11], (6, 15, 11)) < -1   # a window does not fit: refused
