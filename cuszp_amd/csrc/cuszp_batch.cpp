@@ -25,6 +25,7 @@ struct cuszp_batch_plan_s {
     int dir = CUSZP_BATCH_COMPRESS;
     int flags = 0;
     int waves = 4;
+    int window = kLookbackWords;   // lookback words per lane (2 or 4)
     std::vector<cuszp_batch_item_t> src;
     std::vector<Item> items;      // non-empty arrays, in caller order
     uint64_t ntiles = 0;
@@ -115,9 +116,9 @@ int enqueue(const cuszp_batch_plan_s& p, hipStream_t st) {
     BATCH_HIP(launch_bump_epoch(p.d_misc, st));
     const uint32_t n = (uint32_t)p.items.size();
     if (p.dir == CUSZP_BATCH_COMPRESS)
-        BATCH_HIP(launch_compress_batch(p.waves, p.d_items, n, p.ntiles, sc, p.d_misc, st));
+        BATCH_HIP(launch_compress_batch(p.waves, p.window, p.d_items, n, p.ntiles, sc, p.d_misc, st));
     else
-        BATCH_HIP(launch_decompress_batch(p.waves, p.d_items, n, p.ntiles, sc, p.d_misc, st));
+        BATCH_HIP(launch_decompress_batch(p.waves, p.window, p.d_items, n, p.ntiles, sc, p.d_misc, st));
     return CUSZP_SUCCESS;
 }
 
@@ -149,7 +150,7 @@ int capture(cuszp_batch_plan_s& p) {
 // (log2 of its tile count and of its array count); later plans of the same
 // class start from the recorded shape.
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, int> g_tuned;   // -> waves per workgroup
+std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tuned;   // -> (waves per workgroup, lookback window)
 
 std::tuple<int, int, int> size_class(const cuszp_batch_plan_s& p) {
     uint64_t bytes = 0;
@@ -188,7 +189,10 @@ int cuSZp_batch_plan_create(cuszp_batch_plan_t* out, const cuszp_batch_item_t* i
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto f = g_tuned.find(size_class(*p));
-        if (f != g_tuned.end()) p->waves = f->second;
+        if (f != g_tuned.end()) {
+            p->waves = f->second.first;
+            p->window = f->second.second;
+        }
     }
     if (!rc && hipMalloc((void**)&p->d_misc, 8 * sizeof(unsigned long long)) != hipSuccess) rc = CUSZP_ERROR_HIP;
     const size_t n = count ? count : 1;
@@ -252,15 +256,16 @@ const unsigned long long* cuSZp_batch_plan_device_sizes(cuszp_batch_plan_t p) { 
 int cuSZp_batch_plan_tune(cuszp_batch_plan_t p, hipStream_t stream, int* waves_out, unsigned* grid_out) {
     if (!p) return CUSZP_ERROR_INVALID_ARGUMENT;
     if (p->items.empty()) return CUSZP_SUCCESS;
-    const int kWaves[2] = {4, 8};
+    const int kShapes[4][2] = {{4, 4}, {8, 4}, {4, 2}, {8, 2}};   // (waves, lookback window)
     hipEvent_t e0, e1;
     BATCH_HIP(hipEventCreate(&e0));
     BATCH_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_w = p->waves;
+    int best_w = p->waves, best_k = p->window;
     int rc = CUSZP_SUCCESS;
-    for (int w : kWaves) {
-        p->waves = w;
+    for (const auto& shape : kShapes) {
+        p->waves = shape[0];
+        p->window = shape[1];
         rc = build_table(*p, stream);
         if (rc) break;
         rc = enqueue(*p, stream);   // warm-up
@@ -276,18 +281,20 @@ int cuSZp_batch_plan_tune(cuszp_batch_plan_t p, hipStream_t stream, int* waves_o
         hipEventElapsedTime(&ms, e0, e1);
         if (ms < best) {
             best = ms;
-            best_w = w;
+            best_w = shape[0];
+            best_k = shape[1];
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc) return rc;
     p->waves = best_w;
+    p->window = best_k;
     rc = build_table(*p, stream);
     if (rc) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned[size_class(*p)] = best_w;
+        g_tuned[size_class(*p)] = {best_w, best_k};
     }
     if (p->exec || (p->flags & CUSZP_BATCH_GRAPH)) rc = capture(*p);
     // tuning runs leave per-array state from the candidate launches: clear it

@@ -932,7 +932,7 @@ __device__ __forceinline__ bool groups_full(const Geom& geo, uint64_t grp0) {
 // copies its staged payload to the stream with 16-byte stores. A wave whose
 // payload did not fit its staging area runs its chunks again and stores the
 // words directly.
-template <int DIM, int MODE, class T, int WAVES>
+template <int DIM, int MODE, class T, int WAVES, int LB = kLookbackWords>
 __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t tile, const ScanCtx& sc,
                                               SmemC<WAVES>& sm) {
     constexpr int G = ShapeC<T>::G;              // groups per chunk
@@ -1244,7 +1244,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             ttot += t;
         }
         if (wave == 0) {
-            const uint64_t e = wave_lookback<kLookbackWords>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
+            const uint64_t e = wave_lookback<LB>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
                 return compress_tile_bytes<DIM, MODE, T, kChunks * G, WAVES>((const T*)cur.src, geo, r,
                                                                             p - cur.tile0, lane);
             });
@@ -1282,7 +1282,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
 //      stream words (the next chunk's loads are issued while this chunk
 //      decodes), transposes them back, rebuilds q along the Lorenzo chains and
 //      stores (T)q * eb * 2.
-template <int DIM, int MODE, class T, int WAVES>
+template <int DIM, int MODE, class T, int WAVES, int LB = kLookbackWords>
 __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t tile, const ScanCtx& sc,
                                                 SmemD<WAVES>& sm) {
     constexpr int G = ShapeC<T>::G;
@@ -1324,7 +1324,7 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         ttot += t;
     }
     if (wave == 0) {
-        const uint64_t e = wave_lookback<kLookbackWords>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
+        const uint64_t e = wave_lookback<LB>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
             return header_tile_bytes<DIM, MODE, kChunks * G, WAVES>(cmp, geo, p - cur.tile0, lane);
         });
         if (lane == 0) sm.excl = e;
@@ -1616,7 +1616,7 @@ decompress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
     case 17: RUN(3, kOutlier, double); break;                                         \
     default: break;
 
-template <int WAVES>
+template <int WAVES, int LB>
 __global__ void __launch_bounds__(WAVES * kWaveSize, kCompressWavesPerSimd)
 compress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx sc,
                       const unsigned long long* __restrict__ epoch, const uint64_t base) {
@@ -1625,12 +1625,12 @@ compress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx s
     const ItemTable src{items, nitems};
     const uint64_t tile = base + blockIdx.x;
     const Item cur = src.get(src.locate(tile, 0));
-#define CUSZP_RUN_C(D, M, T) compress_tile<D, M, T, WAVES>(cur, tile, sc, sm)
+#define CUSZP_RUN_C(D, M, T) compress_tile<D, M, T, WAVES, LB>(cur, tile, sc, sm)
     switch (cur.variant) { CUSZP_BATCH_CASES(CUSZP_RUN_C) }
 #undef CUSZP_RUN_C
 }
 
-template <int WAVES>
+template <int WAVES, int LB>
 __global__ void __launch_bounds__(WAVES * kWaveSize, kDecompressWavesPerSimd)
 decompress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx sc,
                         const unsigned long long* __restrict__ epoch, const uint64_t base) {
@@ -1639,7 +1639,7 @@ decompress_batch_kernel(const Item* __restrict__ items, uint32_t nitems, ScanCtx
     const ItemTable src{items, nitems};
     const uint64_t tile = base + blockIdx.x;
     const Item cur = src.get(src.locate(tile, 0));
-#define CUSZP_RUN_D(D, M, T) decompress_tile<D, M, T, WAVES>(cur, tile, sc, sm)
+#define CUSZP_RUN_D(D, M, T) decompress_tile<D, M, T, WAVES, LB>(cur, tile, sc, sm)
     switch (cur.variant) { CUSZP_BATCH_CASES(CUSZP_RUN_D) }
 #undef CUSZP_RUN_D
 }

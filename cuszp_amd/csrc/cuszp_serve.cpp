@@ -224,12 +224,12 @@ int lane_launch(cuszp_server_s& s, Lane& L) {
     const size_t nc = L.ncomp;
     if (nc) {
         SRV_HIP(launch_bump_epoch(L.d_misc, L.st));
-        SRV_HIP(launch_compress_batch(kServeWaves, L.d_items, (uint32_t)nc, tc, sc, L.d_misc, L.st));
+        SRV_HIP(launch_compress_batch(kServeWaves, kLookbackWords, L.d_items, (uint32_t)nc, tc, sc, L.d_misc, L.st));
         ++L.launches;
     }
     if (n > nc) {
         SRV_HIP(launch_bump_epoch(L.d_misc, L.st));
-        SRV_HIP(launch_decompress_batch(kServeWaves, L.d_items + nc, (uint32_t)(n - nc), td, sc, L.d_misc, L.st));
+        SRV_HIP(launch_decompress_batch(kServeWaves, kLookbackWords, L.d_items + nc, (uint32_t)(n - nc), td, sc, L.d_misc, L.st));
         ++L.launches;
     }
     SRV_HIP(hipMemcpyAsync(L.h_err, L.d_err, n * sizeof(unsigned int), hipMemcpyDeviceToHost, L.st));

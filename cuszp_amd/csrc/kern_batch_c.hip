@@ -1,4 +1,5 @@
-// Batched compress kernel instantiations (4 and 8 waves per workgroup) and launchers.
+// Batched compress kernel instantiations (4 or 8 waves per workgroup, lookback
+// windows of 2 or 4 words per lane) and launchers.
 #include "launch.hpp"
 
 namespace cuszp {
@@ -9,11 +10,17 @@ __global__ void bump_epoch_kernel(unsigned long long* epoch) {
     if (threadIdx.x == 0) epoch[0] = epoch[0] + 1;
 }
 
-hipError_t launch_compress_batch(int waves, const Item* items, uint32_t nitems, uint64_t ntiles,
+hipError_t launch_compress_batch(int waves, int window, const Item* items, uint32_t nitems, uint64_t ntiles,
                                  const ScanCtx& sc, const unsigned long long* epoch, hipStream_t st) {
     if (!ntiles) return hipSuccess;
-    if (waves == 4) return launch_tiles(compress_batch_kernel<4>, 256, ntiles, st, items, nitems, sc, epoch);
-    if (waves == 8) return launch_tiles(compress_batch_kernel<8>, 512, ntiles, st, items, nitems, sc, epoch);
+#define CUSZP_LAUNCH_C(W, K)                                                                            \
+    if (waves == W && window == K)                                                                    \
+        return launch_tiles(compress_batch_kernel<W, K>, W * kWaveSize, ntiles, st, items, nitems, sc, epoch);
+    CUSZP_LAUNCH_C(4, 2)
+    CUSZP_LAUNCH_C(4, 4)
+    CUSZP_LAUNCH_C(8, 2)
+    CUSZP_LAUNCH_C(8, 4)
+#undef CUSZP_LAUNCH_C
     return hipErrorInvalidValue;
 }
 

@@ -26,10 +26,11 @@ hipError_t launch_decompress_f64(int dim, int mode, int waves, const Item& item,
 
 // Batched launches over a device table of items (ordered by tile0), one
 // workgroup per tile; the scan tag is read from *epoch when the kernel starts
-// (graph-replay safe).
-hipError_t launch_compress_batch(int waves, const Item* items, uint32_t nitems, uint64_t ntiles,
+// (graph-replay safe). waves: 4 or 8 per workgroup; window: lookback words per
+// lane, 2 or 4 (128 or 256 predecessors per round trip).
+hipError_t launch_compress_batch(int waves, int window, const Item* items, uint32_t nitems, uint64_t ntiles,
                                  const ScanCtx& sc, const unsigned long long* epoch, hipStream_t st);
-hipError_t launch_decompress_batch(int waves, const Item* items, uint32_t nitems, uint64_t ntiles,
+hipError_t launch_decompress_batch(int waves, int window, const Item* items, uint32_t nitems, uint64_t ntiles,
                                    const ScanCtx& sc, const unsigned long long* epoch, hipStream_t st);
 // Increments *epoch (one thread); runs before each batched launch.
 hipError_t launch_bump_epoch(unsigned long long* epoch, hipStream_t st);

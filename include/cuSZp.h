@@ -198,8 +198,10 @@ int cuSZp_batch_plan_execute(cuszp_batch_plan_t plan, hipStream_t stream);
 int cuSZp_batch_plan_results(cuszp_batch_plan_t plan, size_t* cmpSizes, int* statuses, hipStream_t stream);
 /* Device array of per-array stream lengths written by each execute (compress). */
 const unsigned long long* cuSZp_batch_plan_device_sizes(cuszp_batch_plan_t plan);
-/* Times the candidate launch shapes on `stream` (runs the batch several times)
- * and keeps the fastest; returns its workgroup size (waves of 64 lanes) in
+/* Times the candidate launch shapes on `stream` (runs the batch several times):
+ * workgroups of 4 or 8 waves of 64 lanes, and lookback windows of 128 or 256
+ * predecessor tiles per round trip; keeps the fastest for this plan and for
+ * later plans of the same size class. Returns its workgroup size (waves) in
  * *waves and the workgroups of its (first) launch, one per tile, in *grid. */
 int cuSZp_batch_plan_tune(cuszp_batch_plan_t plan, hipStream_t stream, int* waves, unsigned* grid);
 void cuSZp_batch_plan_destroy(cuszp_batch_plan_t plan);
