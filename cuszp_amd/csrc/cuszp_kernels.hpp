@@ -1578,7 +1578,9 @@ template <int DIM, int MODE, class T, int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize, kCompressWavesPerSimd)
 compress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
     __shared__ SmemC<WAVES> sm;
-    compress_tile<DIM, MODE, T, WAVES>(item, base + blockIdx.x, sc, sm);
+    // a compress tile's predecessors have mostly published by the time its
+    // payload is staged: the 128-tile window (2 words per lane) is enough
+    compress_tile<DIM, MODE, T, WAVES, 2>(item, base + blockIdx.x, sc, sm);
 }
 
 template <int DIM, int MODE, class T, int WAVES>
