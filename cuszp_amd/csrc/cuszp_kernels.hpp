@@ -658,7 +658,13 @@ struct Meta {
     __device__ __forceinline__ bool sel() const { return (int32_t)a < 0; }
     __device__ __forceinline__ bool nz() const { return b != 0; }
     __device__ __forceinline__ bool has(uint32_t l) const { return (b >> l) & 1; }
+    // lane l's word offset o if the lane carries a word, else `absent`: the
+    // lane's bit sign-extended to a mask (v_bfe_i32) selects with one v_bfi
+    __device__ __forceinline__ uint32_t pick(uint32_t l, uint32_t o, uint32_t absent) const;
 };
+__device__ __forceinline__ uint32_t Meta::pick(uint32_t l, uint32_t o, uint32_t absent) const {
+    return bfi((uint32_t)__builtin_amdgcn_sbfe((int)b, (int)l, 1), o, absent);
+}
 __device__ __forceinline__ Meta make_meta(uint32_t off, const BlockView& v, bool nz) {
     Meta m;
     const bool wide = v.R >= 32;
@@ -1158,7 +1164,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                         if constexpr (MODE == kOutlier) {
                             if (m.sel() && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
                         }
-                        const uint32_t o = m.has(l) ? m.off() + (wideg ? wo_wide : wo) : kStageBytes;
+                        const uint32_t o = m.pick(l, m.off() + (wideg ? wo_wide : wo), kStageBytes);
                         *(lds_u32u*)(sbase + o) = w;
                         if constexpr (DIM != 1) {
                             if (wideg && m.nz() && lane < 2) *(lds_u32u*)(sbase + m.off() + 4 * lane) = sm.aux[wave][2 * g + lane];
