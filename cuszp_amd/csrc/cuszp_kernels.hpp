@@ -1454,7 +1454,9 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         // Chunks go in pairs through two word buffers: a chunk's loads are
         // issued before the stores of the chunk decoded meanwhile (loads and
         // stores retire in order on this GPU, so a load issued after a store
-        // would also wait for that store).
+        // would also wait for that store). The values are stored nontemporally:
+        // the output is written once, and dirty lines of it left in L2 and the
+        // last-level cache would cost the next kernel their write-back.
         Walker wkc;   // 2D/3D: the chunk being decoded
         if constexpr (DIM != 1) wkc.init(wgrp0, geo);
         // 2D/3D output staging: block width BW, rows of RL values, row stride RS
@@ -1507,7 +1509,7 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
             if constexpr (DIM == 1) {
                 T* const orun = out + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
 #pragma unroll
-                for (int g = 0; g < G; ++g) orun[step * g] = value_of(X[g]);
+                for (int g = 0; g < G; ++g) __builtin_nontemporal_store(value_of(X[g]), orun + step * g);
             } else {
                 lds_u8* const stg = (lds_u8*)sm.ostage[wave];
 #pragma unroll
@@ -1520,8 +1522,8 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
                 const v4u v1 = *(__attribute__((address_space(3))) v4u*)(stg + st_r + (64 / VPR) * RS * sizeof(T));
                 __builtin_amdgcn_wave_barrier();
                 T* const ob = out + block_origin<DIM>(geo, wkc);
-                *(v4uu*)(ob + roff) = v0;
-                *(v4uu*)(ob + roff + rdelta) = v1;
+                __builtin_nontemporal_store(v0, (v4uu*)(ob + roff));
+                __builtin_nontemporal_store(v1, (v4uu*)(ob + roff + rdelta));
                 wkc.step(geo, G);
             }
         };
