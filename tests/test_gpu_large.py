@@ -73,3 +73,55 @@ def test_full_size_round_trip_properties(device):
     assert sb == size and torch.equal(cb[:size], c[:size])
     q = P.quality(x, y)
     assert q["max_abs_err"] <= ebf * (1 + 2.0**-20) + 4 * 2.0**-24 * float(x.abs().max())
+
+
+# more than 2^31 elements: every element and stream offset needs 64 bits
+BEYOND_2_31 = [
+    (1, (0, 0, 0), (1 << 31) + 4133, "outlier", 1e-3),     # 8 GiB of f32, ragged last block
+    (3, (2048, 1024, 1025), 2048 * 1024 * 1025, "plain", 1e-3),   # z not a multiple of 4
+]
+
+
+@pytest.mark.parametrize("case", BEYOND_2_31, ids=["1D-outlier-2^31+4133", "3D-plain-2048x1024x1025"])
+def test_beyond_2_31_elements_parity(device, oracle, case):
+    """Byte-identical stream and bit-identical decompression against the
+    oracle for f32 arrays of more than 2^31 elements (generated on the device)."""
+    import torch
+    from cuszp_amd import pycuSZp as P
+
+    dim, dims, n, mode, eb = case
+    print(f"case {case}: generating", flush=True)
+    if dim == 1:
+        g = torch.Generator(device=device).manual_seed(5)
+        x = torch.sin(torch.arange(n, dtype=torch.float32, device=device) * 1e-4)
+        x += torch.randn(n, device=device, generator=g) * 2e-3
+    else:
+        X, Y, Z = dims
+        ax = torch.sin(torch.arange(X, dtype=torch.float32, device=device) * 0.013)
+        ay = torch.cos(torch.arange(Y, dtype=torch.float32, device=device) * 0.007)
+        az = torch.sin(torch.arange(Z, dtype=torch.float32, device=device) * 0.011) * 0.5
+        x = (az[:, None, None] + ay[None, :, None] + ax[None, None, :]).reshape(-1).contiguous()
+    assert x.numel() == n and n > (1 << 31)
+    c, size = P.compress_tensor(x, eb, mode=mode, dim=dim, dims=dims if dim != 1 else None)
+    y = P.decompress_tensor(c, size, n, torch.float32, eb, mode=mode, dim=dim, dims=dims if dim != 1 else None)
+    ebf = float(np.float32(eb))
+    worst = 0.0
+    for s in range(0, n, 1 << 28):   # error bound on the device, in slices
+        e = (x[s:s + (1 << 28)].double() - y[s:s + (1 << 28)].double()).abs()
+        worst = max(worst, float(e.max()))
+    assert worst <= ebf * (1 + 2.0**-20) + 4 * 2.0**-24 * float(x.abs().max())
+    torch.cuda.synchronize()
+    got = c[:size].cpu().numpy().tobytes()
+    del c
+    print(f"case {case}: stream {size} bytes; oracle compress", flush=True)
+    xh = x.cpu().numpy()
+    del x
+    want = oracle.compress(xh, eb, P.MODES[mode], dim, dims, nthreads=oracle.max_threads())
+    assert len(got) == len(want)
+    assert got == want
+    del want
+    print(f"case {case}: oracle decompress", flush=True)
+    yref = oracle.decompress(got, n, np.float32, eb, P.MODES[mode], dim, dims, nthreads=oracle.max_threads())
+    yh = y.cpu().numpy()
+    del y
+    assert np.array_equal(yh.view(np.uint32), yref.view(np.uint32))
