@@ -1001,7 +1001,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                     // then the plane masks
                     static_for<0, G>([&](auto gc) {
                         constexpr int g = decltype(gc)::value;
-                        const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array gives q = 0
+                        const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array (q = -1 if r = inf: masked below)
                         roll_fn(g);
                         uint32_t d;
                         if constexpr (MODE == kFixed) {
@@ -1015,7 +1015,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                             qp = (pl >= 0) ? qp : 0u;
                             d = q - qp;
                         }
-                        if constexpr (MODE != kFixed) d = ((vm >> g) & 1) ? d : 0u;   // residuals of padding are 0
+                        d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
                         const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
                         const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
                         if constexpr (DIM != 1) anywide |= ballot((int32_t)a < 0);
@@ -1055,7 +1055,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                 }
                 static_for<0, G>([&](auto gc) {
                     constexpr int g = decltype(gc)::value;
-                    const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array gives q = 0
+                    const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array (q = -1 if r = inf: masked below)
                     roll_fn(g);
                     uint32_t d;
                     if constexpr (MODE == kFixed) {
@@ -1069,7 +1069,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                         qp = (pl >= 0) ? qp : 0u;
                         d = q - qp;
                     }
-                    if constexpr (MODE != kFixed) d = ((vm >> g) & 1) ? d : 0u;   // residuals of padding are 0
+                    d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
                     const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
                     const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
                     uint64_t m1, m2 = 0;

@@ -116,3 +116,42 @@ def test_rate_zero_blocks_written(device, oracle):
     dref = oracle.decompress(got, x.size, np.float32, 1e-2, 1)
     assert np.array_equal(dgot.view(np.uint32), dref.view(np.uint32))
     assert np.all(dgot[:40_000] == 0) and not np.signbit(dgot[:40_000]).any()
+
+
+# error bounds at the edges of the type's range: smallest normal, subnormal
+# (0.5 / eb overflows to inf, so every nonzero value saturates and 0 * inf is
+# NaN), largest whose double is finite, and beyond
+EB_EDGES = [
+    (np.float32, 2.0**-126), (np.float32, 2.0**125), (np.float32, 1e-40), (np.float32, 3e38),
+    (np.float32, 2.0**-126 * (1 - 2.0**-23)), (np.float32, 2.0**126),
+    (np.float64, 2.0**-1022), (np.float64, 2.0**1021), (np.float64, 1e-310), (np.float64, 1.5e308),
+]
+
+
+@pytest.mark.parametrize("dim", [1, 3], ids=["1D", "3D"])
+@pytest.mark.parametrize("case", EB_EDGES, ids=[f"{np.dtype(d).name}-{e:.3g}" for d, e in EB_EDGES])
+def test_error_bound_range_edges(oracle, dim, case):
+    """Streams byte-identical and decompressions bit-identical to the oracle
+    for error bounds at the edges of the type's range, in arrays of whole
+    tiles plus a ragged tail (1D) or a partial last z-layer of blocks (3D).
+    The padding elements of the ragged tiles must code as q = 0 even when
+    0.5 / eb is inf (src/cuSZp_kernels_1D_f32.cu:113-134 zero them)."""
+    dtype, eb = case
+    rng = np.random.default_rng(17)
+    if dim == 1:
+        n, dims = 4 * 16384 + 100, (0, 0, 0)
+    else:
+        dims = (64, 32, 33)
+        n = dims[0] * dims[1] * dims[2]
+    fmax = float(np.finfo(dtype).max)
+    # values of a few hundred bins where eb allows it, else as large as the type allows
+    amp = min(eb * 600.0, fmax / 2)
+    x = (np.sin(np.arange(n) * 0.01) * amp + rng.standard_normal(n) * min(eb * 3.0, fmax / 8)).astype(dtype)
+    x[::977] = 0
+    for mode in (0, 1, 2):
+        got = gpu_compress(x, eb, mode, dim, dims)
+        want = oracle.compress(x, eb, mode, dim, dims)
+        assert got == want, (mode, len(got), len(want))
+        y = gpu_decompress(got, n, dtype, eb, mode, dim, dims)
+        yref = oracle.decompress(want, n, dtype, eb, mode, dim, dims)
+        assert np.array_equal(y.view(np.uint8), yref.view(np.uint8)), mode
