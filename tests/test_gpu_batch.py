@@ -171,3 +171,25 @@ def test_batch_reports_per_array_status(device):
     bad = [items[1], P.batch_item(b, ob, b.numel(), cap_b, -1.0, 1, (0, 0, 0), 0, 1)]
     with pytest.raises(P.CuSZpError, match="array 1"):
         P.BatchPlan(bad)
+
+
+def test_batch_error_bound_range_edges(device, oracle):
+    """Batched streams at the edges of the error-bound range (a subnormal eb,
+    whose 0.5 / eb is inf, and the smallest normal eb) in ragged arrays of
+    every mode are the oracle's streams: padding elements code as q = 0."""
+    import torch
+    from cuszp_amd import pycuSZp as P
+
+    rng = np.random.default_rng(23)
+    arrays = []
+    for dtype, eb in ((np.float32, 1e-40), (np.float32, 2.0**-126), (np.float64, 1e-310), (np.float64, 2.0**-1022)):
+        for mode in (0, 1, 2):
+            for dim, dims, n in ((1, (0, 0, 0), 40_000 + 37), (3, (36, 20, 7), 36 * 20 * 7)):
+                amp = eb * 600.0
+                a = (np.sin(np.arange(n) * 0.01) * amp + rng.standard_normal(n) * eb * 3.0).astype(dtype)
+                a[::977] = 0
+                arrays.append((a, eb, mode, dim, dims))
+    _, res = _compress_batch(P, torch, arrays)
+    for i, (a, eb, mode, dim, dims) in enumerate(arrays):
+        want = oracle.compress(a, eb, mode, dim, dims)
+        assert res[0][i] == want, f"array {i}: {a.dtype} eb={eb} mode={mode} dim={dim}"
