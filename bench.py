@@ -369,17 +369,15 @@ def gen_field_batch(torch, n, seed, dtype, dev):
     return v.to(dtype)
 
 
-def run_batch(args, np, torch, L, _lib):
-    from cuszp_amd import pycuSZp as P
+def batch_arrays(torch, L, dev, specs=None):
+    """The batch's arrays on the device: dicts with the values x, element count
+    n, absolute bound eb, type (0 f32, 1 f64), mode, worst-case stream size cap
+    and element size s. All arrays are processed as 1D."""
+    from cuszp_amd import _lib
 
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
-    sp = ctypes.c_void_p(stream.cuda_stream)
-    t0 = time.time()
-    specs = batch_specs()
     U0 = _lib.Uint3(0, 0, 0)
     arrs = []
-    for sp_ in specs:
+    for sp_ in (specs or batch_specs()):
         dt = torch.float64 if sp_["f64"] else torch.float32
         if sp_["kind"] == "weight":
             g = torch.Generator(device=dev).manual_seed(sp_["seed"])
@@ -391,10 +389,23 @@ def run_batch(args, np, torch, L, _lib):
             eb = rel_to_abs(torch, x, sp_["rel"])
         typ = 1 if sp_["f64"] else 0
         cap = L.cuSZp_max_compressed_size(sp_["n"], 1, U0, typ, sp_["mode"])
-        arrs.append(dict(x=x, n=sp_["n"], eb=eb, typ=typ, mode=sp_["mode"], cap=cap, s=8 if typ else 4,
-                         cb=torch.empty(cap, dtype=torch.uint8, device=dev),
-                         cp=torch.empty(cap, dtype=torch.uint8, device=dev),
-                         yb=torch.empty_like(x), yp=torch.empty_like(x)))
+        arrs.append(dict(x=x, n=sp_["n"], eb=eb, typ=typ, mode=sp_["mode"], cap=cap, s=8 if typ else 4))
+    return arrs
+
+
+def run_batch(args, np, torch, L, _lib):
+    from cuszp_amd import pycuSZp as P
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    t0 = time.time()
+    U0 = _lib.Uint3(0, 0, 0)
+    arrs = batch_arrays(torch, L, dev)
+    for a in arrs:
+        a.update(cb=torch.empty(a["cap"], dtype=torch.uint8, device=dev),
+                 cp=torch.empty(a["cap"], dtype=torch.uint8, device=dev),
+                 yb=torch.empty_like(a["x"]), yp=torch.empty_like(a["x"]))
     torch.cuda.synchronize()
     raw = sum(a["n"] * a["s"] for a in arrs)
     log(f"[bench] batch of {len(arrs)} arrays, {raw / 1e9:.3f} GB, generated in {time.time() - t0:.1f}s")
@@ -603,12 +614,19 @@ def serve_trace(count=20_000, seed=4):
     return out
 
 
-def run_serve(args, np, torch, L, _lib):
+def serve_setup(np, torch, L, dev, count=20_000):
+    """Sources, stored streams and the request list of the serving workload.
+
+    Sources: one 256 MiB array per (dtype, generator); a request is a slice of
+    one of them. Stored streams: the first 256 compress draws of another seed,
+    compressed once with the single-array entry, are what decompress requests
+    read. Returns a dict: trace, reqs [(direction, BatchItem)], nbytes
+    (uncompressed bytes per request), meta per request (("c", trace entry) or
+    ("d", store index)), store [(stream tensor, size, n, type, eb, mode)],
+    slice_of(trace entry) -> device tensor and eb_of(trace entry)."""
+    from cuszp_amd import _lib
     from cuszp_amd import pycuSZp as P
 
-    dev = torch.device("cuda:0")
-    t0 = time.time()
-    # sources: one 256 MiB array per (dtype, generator); requests are slices of them
     src = {}
     for f64 in (False, True):
         dt = torch.float64 if f64 else torch.float32
@@ -618,7 +636,7 @@ def run_serve(args, np, torch, L, _lib):
         f = gen_field_batch(torch, n, 30 + f64, dt, dev)
         src[(f64, "weight")] = (w, None)
         src[(f64, "field")] = (f, (f.max().item(), f.min().item()))
-    trace = serve_trace()
+    trace = serve_trace(count=count)
 
     def eb_of(r):
         if r["kind"] == "weight":
@@ -633,8 +651,6 @@ def run_serve(args, np, torch, L, _lib):
         st = int(r["pos"] * (x.numel() - r["n"]))
         return x[st:st + r["n"]]
 
-    # stored streams for the decompress requests: the first 256 compress draws of
-    # another seed, compressed once with the single-array entry
     store = []
     for r in serve_trace(count=256, seed=5):
         x = slice_of(r)
@@ -647,17 +663,32 @@ def run_serve(args, np, torch, L, _lib):
         del c
     torch.cuda.synchronize()
     rng = np.random.default_rng(6)
-    reqs, nbytes = [], []
+    reqs, nbytes, meta = [], [], []
     for r in trace:
         if r["comp"]:
             x = slice_of(r)
             typ = 1 if r["f64"] else 0
             reqs.append([P.BATCH_COMPRESS, P.batch_item(x, None, r["n"], 0, eb_of(r), 1, (0, 0, 0), typ, r["mode"])])
             nbytes.append(r["n"] * (8 if typ else 4))
+            meta.append(("c", r))
         else:
-            c, size, n, typ, eb, mode = store[int(rng.integers(0, len(store)))]
+            k = int(rng.integers(0, len(store)))
+            c, size, n, typ, eb, mode = store[k]
             reqs.append([P.BATCH_DECOMPRESS, P.batch_item(c, None, n, size, eb, 1, (0, 0, 0), typ, mode)])
             nbytes.append(n * (8 if typ else 4))
+            meta.append(("d", k))
+    return dict(src=src, trace=trace, reqs=reqs, nbytes=nbytes, meta=meta, store=store, slice_of=slice_of,
+                eb_of=eb_of)
+
+
+def run_serve(args, np, torch, L, _lib):
+    from cuszp_amd import pycuSZp as P
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    setup = serve_setup(np, torch, L, dev)
+    trace, reqs, nbytes = setup["trace"], setup["reqs"], setup["nbytes"]
+    slice_of, store, eb_of = setup["slice_of"], setup["store"], setup["eb_of"]
     total = float(sum(nbytes))
     mean_b = total / len(reqs)
     pool = torch.empty(64_000_000_000, dtype=torch.uint8, device=dev)
@@ -697,7 +728,7 @@ def run_serve(args, np, torch, L, _lib):
                          "sequential_p99_ms": round(pct(ls, 99) * 1e3, 3)})
             log(f"[bench] load {load}: {rows[-1]}")
     top = rows[-1]
-    cpu = None if args.no_cpu_baseline else cpu_baseline_serve(np, reqs, trace, slice_of, store, eb_of, rng,
+    cpu = None if args.no_cpu_baseline else cpu_baseline_serve(np, reqs, trace, slice_of, store, eb_of,
                                                                args.cpu_sample_seconds)
     out = {
         "metric": METRIC,
@@ -741,7 +772,7 @@ def run_serve(args, np, torch, L, _lib):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline_serve(np, reqs, trace, slice_of, store, eb_of, rng, budget_s):
+def cpu_baseline_serve(np, reqs, trace, slice_of, store, eb_of, budget_s):
     """Oracle (C++/OpenMP, all host threads) over the trace's compress requests
     in order, until about budget_s seconds have been spent."""
     from oracle import oracle as O

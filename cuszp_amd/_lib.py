@@ -35,6 +35,9 @@ EXPORTED = (
         "cuSZp_compress_async",
         "cuSZp_decompress_async",
         "cuSZp_last_async_status",
+        "cuSZp_scan_report",
+        "cuSZp_scan_config",
+        "cuSZp_batch_plan_scan_report",
         "cuSZp_status_string",
         "cuSZp_version",
         "cuSZp_batch_plan_create",
@@ -72,6 +75,22 @@ class BatchItem(ctypes.Structure):
         ("type", ctypes.c_int),
         ("mode", ctypes.c_int),
     ]
+
+
+class ScanReport(ctypes.Structure):
+    """cuszp_scan_report_t (include/cuSZp.h, scan diagnostics)."""
+
+    _fields_ = [
+        ("info", ctypes.c_uint),
+        ("polls", ctypes.c_ulonglong),
+        ("waits", ctypes.c_ulonglong),
+        ("claims", ctypes.c_ulonglong),
+        ("wait_ticks", ctypes.c_ulonglong),
+    ]
+
+
+SCAN_INFO_STOLEN = 8
+WARNING_BOUND_EXCEEDED = 8
 
 
 class Completion(ctypes.Structure):
@@ -137,6 +156,12 @@ def load() -> ctypes.CDLL:
     L.cuSZp_decompress_async.restype = i32
     L.cuSZp_last_async_status.argtypes = [vp]
     L.cuSZp_last_async_status.restype = i32
+    L.cuSZp_scan_report.argtypes = [vp, ctypes.POINTER(ScanReport)]
+    L.cuSZp_scan_report.restype = i32
+    L.cuSZp_scan_config.argtypes = [i32, ctypes.c_uint, ctypes.c_uint, i32]
+    L.cuSZp_scan_config.restype = i32
+    L.cuSZp_batch_plan_scan_report.argtypes = [vp, vp, ctypes.POINTER(ScanReport)]
+    L.cuSZp_batch_plan_scan_report.restype = i32
     L.cuSZp_status_string.argtypes = [i32]
     L.cuSZp_status_string.restype = ctypes.c_char_p
     L.cuSZp_version.argtypes = []

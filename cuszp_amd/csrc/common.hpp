@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -14,13 +15,54 @@ namespace cuszp {
 
 constexpr uint64_t kMaxStream = (1ull << 38) - 1;   // status-word value field
 
+// Status of a launch's (or an array's) sticky bits: errors first, then the
+// lossy-block warning; kInfoStolen is information only.
 inline int status_from_bits(unsigned long long bits) {
-    bits &= kErrMask;
     if (bits & kErrStall) return CUSZP_ERROR_INTERNAL;
     if (bits & kErrOverflow) return CUSZP_ERROR_OUTPUT_TOO_SMALL;
     if (bits & kErrTruncated) return CUSZP_ERROR_CORRUPT_STREAM;
+    if (bits & kWarnLossy) return CUSZP_WARNING_BOUND_EXCEEDED;
     return CUSZP_SUCCESS;
 }
+// A status after which the output is complete (a warning still delivers it).
+inline bool status_delivers(int st) { return st == CUSZP_SUCCESS || st == CUSZP_WARNING_BOUND_EXCEEDED; }
+
+// Scan settings of later launches (cuSZp_scan_config), shared by every entry.
+struct ScanSettings {
+    std::atomic<uint32_t> steal_ticks{(uint32_t)kStealTicks};
+    std::atomic<uint32_t> hold_every{0};
+    std::atomic<uint32_t> hold_ticks{0};
+    std::atomic<uint32_t> stats{0};
+};
+inline ScanSettings& scan_settings() {
+    static ScanSettings s;
+    return s;
+}
+
+// Scan context of a launch over `status` (kStatusGuard guard words in front
+// are the caller's), with the launch's error word and the diagnostic words
+// diag[0..7].
+inline ScanCtx make_scan_ctx(unsigned long long* status, unsigned int* err, unsigned long long* diag,
+                             unsigned long long tag) {
+    const ScanSettings& k = scan_settings();
+    ScanCtx c;
+    c.status = status;
+    c.err = err;
+    c.diag = diag;
+    c.tag = tag;
+    c.steal_ticks = k.steal_ticks.load(std::memory_order_relaxed);
+    c.hold_every = k.hold_every.load(std::memory_order_relaxed);
+    c.hold_ticks = k.hold_ticks.load(std::memory_order_relaxed);
+    c.stats = k.stats.load(std::memory_order_relaxed);
+    if (!c.hold_ticks) c.hold_every = 0;
+    return c;
+}
+
+// Device words of a launch's scan state besides the status array:
+// [0] epoch (batched launches), [1] stream length (blocking entries),
+// [2] sticky bits, [4..7] watchdog diagnostics, [8..11] lookback counters.
+constexpr int kMiscWords = 16;
+constexpr int kMiscErr = 2, kMiscDiag = 4, kMiscCounters = 8;
 
 // Geometry of a (dim, nbEle, dims) triple; mirrors the reference's block counts.
 inline int make_geom(int dim, size_t n, cuszp_dims_t d, Geom& g) {

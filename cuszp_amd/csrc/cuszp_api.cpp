@@ -31,11 +31,12 @@ namespace {
 
 struct Slot {
     std::mutex mu;
-    unsigned long long* d_status = nullptr;   // tile status words
+    unsigned long long* d_status = nullptr;   // tile status words (kStatusGuard guard words first)
     uint64_t cap_tiles = 0;
-    unsigned long long* d_misc = nullptr;     // [1] total, [2] error bits
-    unsigned long long* h_res = nullptr;      // pinned copy of d_misc[1..3)
+    unsigned long long* d_misc = nullptr;     // kMiscWords: stream length, sticky bits, diagnostics, counters
+    unsigned long long* h_res = nullptr;      // pinned host copies of d_misc words
     uint32_t epoch = 0;
+    unsigned info = 0;                        // information bits harvested since the last scan report
 };
 
 std::mutex g_pool_mu;
@@ -55,26 +56,39 @@ Slot& slot_for(int dev, hipStream_t st) {
     } while (0)
 
 // Make room for `ntiles` status words and advance the epoch. Caller holds s.mu.
+// A failed allocation leaves the slot consistent: either the old buffer and
+// capacity are kept, or there is no buffer and the capacity is 0.
 int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
     if (!s.d_misc) {
-        CUSZP_HIP(hipMalloc((void**)&s.d_misc, 8 * sizeof(unsigned long long)));
-        CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, 8 * sizeof(unsigned long long), st));
-        CUSZP_HIP(hipHostMalloc((void**)&s.h_res, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+        unsigned long long* m = nullptr;
+        CUSZP_HIP(hipMalloc((void**)&m, kMiscWords * sizeof(unsigned long long)));
+        if (hipHostMalloc((void**)&s.h_res, kMiscWords * sizeof(unsigned long long), hipHostMallocDefault) !=
+            hipSuccess) {
+            hipFree(m);
+            s.h_res = nullptr;
+            return CUSZP_ERROR_HIP;
+        }
+        s.d_misc = m;
+        CUSZP_HIP(hipMemsetAsync(s.d_misc, 0, kMiscWords * sizeof(unsigned long long), st));
     }
     if (ntiles > s.cap_tiles) {
-        if (s.d_status) {
-            CUSZP_HIP(hipStreamSynchronize(st));
-            CUSZP_HIP(hipFree(s.d_status));
-            s.d_status = nullptr;
-        }
         uint64_t cap = s.cap_tiles * 2;
         if (cap < ntiles) cap = ntiles;
         if (cap < 4096) cap = 4096;
+        if (s.d_status) {
+            // the old array may still be read by queued launches on this stream
+            CUSZP_HIP(hipStreamSynchronize(st));
+            CUSZP_HIP(hipFree(s.d_status));
+            s.d_status = nullptr;
+            s.cap_tiles = 0;
+        }
         // kStatusGuard words in front of the status words (lookback windows of
         // the first tiles read there)
-        CUSZP_HIP(hipMalloc((void**)&s.d_status, (kStatusGuard + cap) * sizeof(unsigned long long)));
-        CUSZP_HIP(hipMemsetAsync(s.d_status, 0, (kStatusGuard + cap) * sizeof(unsigned long long), st));
+        unsigned long long* d = nullptr;
+        CUSZP_HIP(hipMalloc((void**)&d, (kStatusGuard + cap) * sizeof(unsigned long long)));
+        s.d_status = d;
         s.cap_tiles = cap;
+        CUSZP_HIP(hipMemsetAsync(s.d_status, 0, (kStatusGuard + cap) * sizeof(unsigned long long), st));
         s.epoch = 0;
     }
     if (++s.epoch >= (1u << 24)) {   // 24-bit epoch tag wrapped: clear stale words
@@ -84,31 +98,26 @@ int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
     return CUSZP_SUCCESS;
 }
 
-
-// d_misc: [1] total of blocking calls, [2] sticky error bits (low 32 bits),
-// [4..6] watchdog diagnostics.
 ScanCtx scan_ctx(const Slot& s) {
-    ScanCtx c;
-    c.status = s.d_status + kStatusGuard;
-    c.err = (unsigned int*)(s.d_misc + 2);
-    c.diag = s.d_misc + 4;
-    c.tag = (unsigned long long)s.epoch << kEpochShift;
-    return c;
+    return make_scan_ctx(s.d_status + kStatusGuard, (unsigned int*)(s.d_misc + kMiscErr), s.d_misc + kMiscDiag,
+                         (unsigned long long)s.epoch << kEpochShift);
 }
 
-// Read {total, error bits} after a blocking launch; clear errors once seen.
+// Read {total, sticky bits} after a blocking launch (or for the async status);
+// clear the bits once seen and keep the information bits for the scan report.
 int finish_blocking(Slot& s, hipStream_t st, size_t* total) {
     CUSZP_HIP(hipMemcpyAsync(s.h_res, s.d_misc + 1, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     CUSZP_HIP(hipStreamSynchronize(st));
     const unsigned long long bits = s.h_res[1] & 0xffffffffull;
     if (total) *total = (size_t)s.h_res[0];
     if (bits & kErrStall) {
-        CUSZP_HIP(hipMemcpy(s.h_res + 4, s.d_misc + 4, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        CUSZP_HIP(hipMemcpy(s.h_res + 4, s.d_misc + kMiscDiag, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         std::fprintf(stderr, "cuSZp: internal error: tile %llu stopped waiting for tile %llu (status word 0x%016llx)\n",
                      s.h_res[4] - 1, s.h_res[5], s.h_res[6]);
     }
     if (bits) {
-        CUSZP_HIP(hipMemsetAsync(s.d_misc + 2, 0, 6 * sizeof(unsigned long long), st));
+        s.info |= (unsigned)(bits & kInfoStolen);
+        CUSZP_HIP(hipMemsetAsync(s.d_misc + kMiscErr, 0, (kMiscCounters - kMiscErr) * sizeof(unsigned long long), st));
         CUSZP_HIP(hipStreamSynchronize(st));
     }
     return status_from_bits(bits);
@@ -137,11 +146,6 @@ TilePlan plan_tiles(int dim, int type, bool compress, const Geom& g) {
 }
 
 namespace {
-
-
-
-
-
 
 int compress_once(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
                   unsigned long long* d_size, double eb, int dim, cuszp_dims_t dims, int type, int mode,
@@ -206,17 +210,6 @@ int decompress_once(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
     return finish_blocking(s, st, nullptr);
 }
 
-int compress_impl(const void* in, unsigned char* out, size_t cap, size_t n, size_t* cmp_size,
-                  unsigned long long* d_size, double eb, int dim, cuszp_dims_t dims, int type, int mode,
-                  hipStream_t st, bool blocking) {
-    return compress_once(in, out, cap, n, cmp_size, d_size, eb, dim, dims, type, mode, st, blocking);
-}
-
-int decompress_impl(void* out, const unsigned char* cmp, size_t n, size_t cmp_size, double eb, int dim,
-                    cuszp_dims_t dims, int type, int mode, hipStream_t st, bool blocking) {
-    return decompress_once(out, cmp, n, cmp_size, eb, dim, dims, type, mode, st, blocking);
-}
-
 void report(const char* fn, int rc) {
     if (rc != CUSZP_SUCCESS) std::fprintf(stderr, "cuSZp: %s: %s\n", fn, cuSZp_status_string(rc));
 }
@@ -240,6 +233,9 @@ const char* cuSZp_status_string(int status) {
         case CUSZP_ERROR_CORRUPT_STREAM: return "compressed stream is shorter than its header describes";
         case CUSZP_ERROR_HIP: return "HIP runtime error";
         case CUSZP_ERROR_INTERNAL: return "internal error: the device-wide scan stalled (output invalid)";
+        case CUSZP_WARNING_BOUND_EXCEEDED:
+            return "warning: a 2D/3D outlier-mode block holds a residual of magnitude 2^31 (saturated codes); "
+                   "it decodes to zeros, beyond the error bound";
         default: return "unknown status";
     }
 }
@@ -258,28 +254,28 @@ size_t cuSZp_max_compressed_size(size_t nbEle, cuszp_dim_t dim, cuszp_dims_t dim
 int cuSZp_compress_ex(const void* d_oriData, unsigned char* d_cmpBytes, size_t cmpCapacity, size_t nbEle,
                       size_t* cmpSize, double errorBound, cuszp_dim_t dim, cuszp_dims_t dims,
                       cuszp_type_t type, cuszp_mode_t mode, hipStream_t stream) {
-    return compress_impl(d_oriData, d_cmpBytes, cmpCapacity, nbEle, cmpSize, nullptr, errorBound, dim, dims,
+    return compress_once(d_oriData, d_cmpBytes, cmpCapacity, nbEle, cmpSize, nullptr, errorBound, dim, dims,
                          type, mode, stream, true);
 }
 
 int cuSZp_decompress_ex(void* d_decData, const unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize,
                         double errorBound, cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type,
                         cuszp_mode_t mode, hipStream_t stream) {
-    return decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, errorBound, dim, dims, type, mode, stream,
+    return decompress_once(d_decData, d_cmpBytes, nbEle, cmpSize, errorBound, dim, dims, type, mode, stream,
                            true);
 }
 
 int cuSZp_compress_async(const void* d_oriData, unsigned char* d_cmpBytes, size_t cmpCapacity, size_t nbEle,
                          unsigned long long* d_cmpSize, double errorBound, cuszp_dim_t dim, cuszp_dims_t dims,
                          cuszp_type_t type, cuszp_mode_t mode, hipStream_t stream) {
-    return compress_impl(d_oriData, d_cmpBytes, cmpCapacity, nbEle, nullptr, d_cmpSize, errorBound, dim, dims,
+    return compress_once(d_oriData, d_cmpBytes, cmpCapacity, nbEle, nullptr, d_cmpSize, errorBound, dim, dims,
                          type, mode, stream, false);
 }
 
 int cuSZp_decompress_async(void* d_decData, const unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize,
                            double errorBound, cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type,
                            cuszp_mode_t mode, hipStream_t stream) {
-    return decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, errorBound, dim, dims, type, mode, stream,
+    return decompress_once(d_decData, d_cmpBytes, nbEle, cmpSize, errorBound, dim, dims, type, mode, stream,
                            false);
 }
 
@@ -292,18 +288,50 @@ int cuSZp_last_async_status(hipStream_t stream) {
     return finish_blocking(s, stream, nullptr);
 }
 
+int cuSZp_scan_report(hipStream_t stream, cuszp_scan_report_t* report) {
+    if (!report) return CUSZP_ERROR_INVALID_ARGUMENT;
+    std::memset(report, 0, sizeof(*report));
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CUSZP_ERROR_HIP;
+    Slot& s = slot_for(dev, stream);
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (!s.d_misc) return CUSZP_SUCCESS;
+    CUSZP_HIP(hipMemcpyAsync(s.h_res, s.d_misc, kMiscWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    CUSZP_HIP(hipStreamSynchronize(stream));
+    report->info = s.info | (unsigned)(s.h_res[kMiscErr] & kInfoStolen);
+    report->polls = s.h_res[kMiscCounters + 0];
+    report->waits = s.h_res[kMiscCounters + 1];
+    report->claims = s.h_res[kMiscCounters + 2];
+    report->wait_ticks = s.h_res[kMiscCounters + 3];
+    s.info = 0;
+    CUSZP_HIP(hipMemsetAsync(s.d_misc + kMiscCounters, 0, 4 * sizeof(unsigned long long), stream));
+    CUSZP_HIP(hipStreamSynchronize(stream));
+    return CUSZP_SUCCESS;
+}
+
+int cuSZp_scan_config(int steal_ticks, unsigned int hold_every, unsigned int hold_ticks, int stats) {
+    // holds are bounded (100 ms) so that a stressed launch still ends promptly
+    if (hold_ticks > 10000000u) return CUSZP_ERROR_INVALID_ARGUMENT;
+    ScanSettings& k = scan_settings();
+    k.steal_ticks = steal_ticks < 0 ? (uint32_t)kStealTicks : (uint32_t)steal_ticks;
+    k.hold_every = hold_ticks ? hold_every : 0;
+    k.hold_ticks = hold_ticks;
+    k.stats = stats ? 1u : 0u;
+    return CUSZP_SUCCESS;
+}
+
 // ---- generic drop-ins: src/cuSZp.cu:4-83, :86-165 ------------------------
 void cuSZp_compress(void* d_oriData, unsigned char* d_cmpBytes, size_t nbEle, size_t* cmpSize, float errorBound,
                     cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode, hipStream_t stream) {
     // f64 receives the float error bound widened to double (src/cuSZp.cu:45)
-    report("cuSZp_compress", compress_impl(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr,
+    report("cuSZp_compress", compress_once(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr,
                                            (double)errorBound, dim, dims, type, mode, stream, true));
 }
 
 void cuSZp_decompress(void* d_decData, unsigned char* d_cmpBytes, size_t nbEle, size_t cmpSize, float errorBound,
                       cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
                       hipStream_t stream) {
-    report("cuSZp_decompress", decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, dim,
+    report("cuSZp_decompress", decompress_once(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, dim,
                                                dims, type, mode, stream, true));
 }
 
@@ -312,13 +340,13 @@ void cuSZp_decompress(void* d_decData, unsigned char* d_cmpBytes, size_t nbEle, 
     void cuSZp_compress_1D_##MODE_NAME##_##TNAME(T* d_oriData, unsigned char* d_cmpBytes, size_t nbEle,     \
                                                   size_t* cmpSize, T errorBound, hipStream_t stream) {      \
         report("cuSZp_compress_1D_" #MODE_NAME "_" #TNAME,                                                  \
-               compress_impl(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr, (double)errorBound, \
+               compress_once(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr, (double)errorBound, \
                              CUSZP_DIM_1D, kNoDims, TENUM, MODE_ENUM, stream, true));                       \
     }                                                                                                       \
     void cuSZp_decompress_1D_##MODE_NAME##_##TNAME(T* d_decData, unsigned char* d_cmpBytes, size_t nbEle,   \
                                                     size_t cmpSize, T errorBound, hipStream_t stream) {     \
         report("cuSZp_decompress_1D_" #MODE_NAME "_" #TNAME,                                                \
-               decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, CUSZP_DIM_1D,      \
+               decompress_once(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, CUSZP_DIM_1D,      \
                                kNoDims, TENUM, MODE_ENUM, stream, true));                                   \
     }
 
@@ -327,14 +355,14 @@ void cuSZp_decompress(void* d_decData, unsigned char* d_cmpBytes, size_t nbEle, 
                                                         size_t nbEle, size_t* cmpSize, cuszp_dims_t dims,   \
                                                         T errorBound, hipStream_t stream) {                 \
         report("cuSZp_compress_" #DIMN "_" #MODE_NAME "_" #TNAME,                                           \
-               compress_impl(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr, (double)errorBound, \
+               compress_once(d_oriData, d_cmpBytes, (size_t)-1, nbEle, cmpSize, nullptr, (double)errorBound, \
                              DIM_ENUM, dims, TENUM, MODE_ENUM, stream, true));                              \
     }                                                                                                       \
     void cuSZp_decompress_##DIMN##_##MODE_NAME##_##TNAME(T* d_decData, unsigned char* d_cmpBytes,           \
                                                           size_t nbEle, size_t cmpSize, cuszp_dims_t dims,  \
                                                           T errorBound, hipStream_t stream) {               \
         report("cuSZp_decompress_" #DIMN "_" #MODE_NAME "_" #TNAME,                                         \
-               decompress_impl(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, DIM_ENUM, dims,    \
+               decompress_once(d_decData, d_cmpBytes, nbEle, cmpSize, (double)errorBound, DIM_ENUM, dims,    \
                                TENUM, MODE_ENUM, stream, true));                                            \
     }
 

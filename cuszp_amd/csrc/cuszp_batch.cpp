@@ -1,4 +1,5 @@
-// Batched C ABI: many arrays in one persistent launch (compress or decompress),
+// Batched C ABI: many arrays in one launch (compress or decompress) with one
+// workgroup per tile of the concatenated arrays,
 // optionally held as a captured hipGraph, with a launch-shape autotuner.
 //
 // A plan validates every array once (the same checks as cuSZp_compress_ex /
@@ -11,6 +12,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -33,12 +35,13 @@ struct cuszp_batch_plan_s {
     size_t cap_items = 0;
     unsigned long long* d_status = nullptr;
     uint64_t cap_tiles = 0;
-    unsigned long long* d_misc = nullptr;   // [0] epoch, [2] launch error bits, [4..6] diagnostics
+    unsigned long long* d_misc = nullptr;   // kMiscWords: [0] epoch, [2] launch bits, [4..7] diagnostics, [8..11] counters
     unsigned int* d_err = nullptr;          // per array
     unsigned long long* d_sizes = nullptr;  // per array (compress)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    unsigned info = 0;                      // information bits harvested since the last scan report
 };
 
 namespace {
@@ -83,18 +86,26 @@ int build_table(cuszp_batch_plan_s& p, hipStream_t st) {
         p.items.push_back(it);
     }
     p.ntiles = tile0;
+    // Growing a buffer: the old one is released before the new one is
+    // allocated, and the capacity drops to 0 in between, so a failed
+    // allocation leaves no buffer that a later call would take as large enough
+    // (every launch of the plan has finished: the table is built only between
+    // synchronised calls).
     if (p.items.size() > p.cap_items) {
         if (p.d_items) BATCH_HIP(hipFree(p.d_items));
         p.d_items = nullptr;
+        p.cap_items = 0;
         BATCH_HIP(hipMalloc((void**)&p.d_items, p.items.size() * sizeof(Item)));
         p.cap_items = p.items.size();
     }
-    if (p.ntiles > p.cap_tiles) {
+    const uint64_t need = p.ntiles ? p.ntiles : 1;   // an empty table still gets a (cleared) status array
+    if (need > p.cap_tiles) {
         if (p.d_status) BATCH_HIP(hipFree(p.d_status));
         p.d_status = nullptr;
+        p.cap_tiles = 0;
         // kStatusGuard words in front (lookback windows of first tiles read there)
-        BATCH_HIP(hipMalloc((void**)&p.d_status, (kStatusGuard + p.ntiles) * sizeof(unsigned long long)));
-        p.cap_tiles = p.ntiles;
+        BATCH_HIP(hipMalloc((void**)&p.d_status, (kStatusGuard + need) * sizeof(unsigned long long)));
+        p.cap_tiles = need;
     }
     // stale status words must not carry the current epoch: start from zero and
     // restart the epoch whenever the table changes
@@ -108,11 +119,9 @@ int build_table(cuszp_batch_plan_s& p, hipStream_t st) {
 
 int enqueue(const cuszp_batch_plan_s& p, hipStream_t st) {
     if (p.items.empty()) return CUSZP_SUCCESS;
-    ScanCtx sc;
-    sc.status = p.d_status + kStatusGuard;
-    sc.err = (unsigned int*)(p.d_misc + 2);
-    sc.diag = p.d_misc + 4;
-    sc.tag = 0;   // the kernels read the epoch word
+    // the kernels read the epoch word for the tag
+    const ScanCtx sc = make_scan_ctx(p.d_status + kStatusGuard, (unsigned int*)(p.d_misc + kMiscErr),
+                                     p.d_misc + kMiscDiag, 0);
     BATCH_HIP(launch_bump_epoch(p.d_misc, st));
     const uint32_t n = (uint32_t)p.items.size();
     if (p.dir == CUSZP_BATCH_COMPRESS)
@@ -194,11 +203,12 @@ int cuSZp_batch_plan_create(cuszp_batch_plan_t* out, const cuszp_batch_item_t* i
             p->window = f->second.second;
         }
     }
-    if (!rc && hipMalloc((void**)&p->d_misc, 8 * sizeof(unsigned long long)) != hipSuccess) rc = CUSZP_ERROR_HIP;
+    if (!rc && hipMalloc((void**)&p->d_misc, kMiscWords * sizeof(unsigned long long)) != hipSuccess) rc = CUSZP_ERROR_HIP;
     const size_t n = count ? count : 1;
     if (!rc && hipMalloc((void**)&p->d_err, n * sizeof(unsigned int)) != hipSuccess) rc = CUSZP_ERROR_HIP;
     if (!rc && hipMalloc((void**)&p->d_sizes, n * sizeof(unsigned long long)) != hipSuccess) rc = CUSZP_ERROR_HIP;
-    if (!rc && hipMemsetAsync(p->d_misc, 0, 8 * sizeof(unsigned long long), stream) != hipSuccess) rc = CUSZP_ERROR_HIP;
+    if (!rc && hipMemsetAsync(p->d_misc, 0, kMiscWords * sizeof(unsigned long long), stream) != hipSuccess)
+        rc = CUSZP_ERROR_HIP;
     if (!rc && hipMemsetAsync(p->d_err, 0, n * sizeof(unsigned int), stream) != hipSuccess) rc = CUSZP_ERROR_HIP;
     if (!rc && hipMemsetAsync(p->d_sizes, 0, n * sizeof(unsigned long long), stream) != hipSuccess) rc = CUSZP_ERROR_HIP;
     if (!rc) rc = build_table(*p, stream);
@@ -226,13 +236,14 @@ int cuSZp_batch_plan_results(cuszp_batch_plan_t p, size_t* cmpSizes, int* status
     const size_t n = p->src.size();
     std::vector<unsigned long long> sizes(n ? n : 1);
     std::vector<unsigned int> errs(n ? n : 1);
-    unsigned long long misc[8];
+    unsigned long long misc[kMiscWords];
     BATCH_HIP(hipMemcpyAsync(misc, p->d_misc, sizeof(misc), hipMemcpyDeviceToHost, stream));
     BATCH_HIP(hipMemcpyAsync(errs.data(), p->d_err, errs.size() * sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
     BATCH_HIP(hipMemcpyAsync(sizes.data(), p->d_sizes, sizes.size() * sizeof(unsigned long long),
                              hipMemcpyDeviceToHost, stream));
     BATCH_HIP(hipStreamSynchronize(stream));
-    const unsigned long long launch_bits = misc[2] & 0xffffffffull;
+    const unsigned long long launch_bits = misc[kMiscErr] & 0xffffffffull;
+    p->info |= (unsigned)(launch_bits & kInfoStolen);
     if (launch_bits & kErrStall)
         std::fprintf(stderr, "cuSZp: internal error: batched tile %llu stopped waiting for tile %llu\n", misc[4] - 1,
                      misc[5]);
@@ -240,11 +251,12 @@ int cuSZp_batch_plan_results(cuszp_batch_plan_t p, size_t* cmpSizes, int* status
     for (size_t i = 0; i < n; ++i) {
         const int st = status_from_bits(errs[i] | (launch_bits & kErrStall));
         if (statuses) statuses[i] = st;
-        if (cmpSizes) cmpSizes[i] = (p->dir == CUSZP_BATCH_COMPRESS && !st) ? (size_t)sizes[i] : 0;
+        if (cmpSizes) cmpSizes[i] = (p->dir == CUSZP_BATCH_COMPRESS && status_delivers(st)) ? (size_t)sizes[i] : 0;
         if (st && !first) first = st;
     }
     if (launch_bits) {
-        BATCH_HIP(hipMemsetAsync(p->d_misc + 2, 0, 6 * sizeof(unsigned long long), stream));
+        BATCH_HIP(hipMemsetAsync(p->d_misc + kMiscErr, 0, (kMiscCounters - kMiscErr) * sizeof(unsigned long long),
+                                 stream));
     }
     if (first) BATCH_HIP(hipMemsetAsync(p->d_err, 0, (n ? n : 1) * sizeof(unsigned int), stream));
     BATCH_HIP(hipStreamSynchronize(stream));
@@ -252,6 +264,23 @@ int cuSZp_batch_plan_results(cuszp_batch_plan_t p, size_t* cmpSizes, int* status
 }
 
 const unsigned long long* cuSZp_batch_plan_device_sizes(cuszp_batch_plan_t p) { return p ? p->d_sizes : nullptr; }
+
+int cuSZp_batch_plan_scan_report(cuszp_batch_plan_t p, hipStream_t stream, cuszp_scan_report_t* report) {
+    if (!p || !report) return CUSZP_ERROR_INVALID_ARGUMENT;
+    std::memset(report, 0, sizeof(*report));
+    unsigned long long misc[kMiscWords];
+    BATCH_HIP(hipMemcpyAsync(misc, p->d_misc, sizeof(misc), hipMemcpyDeviceToHost, stream));
+    BATCH_HIP(hipStreamSynchronize(stream));
+    report->info = p->info | (unsigned)(misc[kMiscErr] & kInfoStolen);
+    report->polls = misc[kMiscCounters + 0];
+    report->waits = misc[kMiscCounters + 1];
+    report->claims = misc[kMiscCounters + 2];
+    report->wait_ticks = misc[kMiscCounters + 3];
+    p->info = 0;
+    BATCH_HIP(hipMemsetAsync(p->d_misc + kMiscCounters, 0, 4 * sizeof(unsigned long long), stream));
+    BATCH_HIP(hipStreamSynchronize(stream));
+    return CUSZP_SUCCESS;
+}
 
 int cuSZp_batch_plan_tune(cuszp_batch_plan_t p, hipStream_t stream, int* waves_out, unsigned* grid_out) {
     if (!p) return CUSZP_ERROR_INVALID_ARGUMENT;
@@ -300,7 +329,8 @@ int cuSZp_batch_plan_tune(cuszp_batch_plan_t p, hipStream_t stream, int* waves_o
     // tuning runs leave per-array state from the candidate launches: clear it
     const size_t n = p->src.size() ? p->src.size() : 1;
     if (!rc && hipMemsetAsync(p->d_err, 0, n * sizeof(unsigned int), stream) != hipSuccess) rc = CUSZP_ERROR_HIP;
-    if (!rc && hipMemsetAsync(p->d_misc + 2, 0, 6 * sizeof(unsigned long long), stream) != hipSuccess)
+    if (!rc && hipMemsetAsync(p->d_misc + kMiscErr, 0, (kMiscWords - kMiscErr) * sizeof(unsigned long long),
+                              stream) != hipSuccess)
         rc = CUSZP_ERROR_HIP;
     if (waves_out) *waves_out = best_w;
     if (grid_out) *grid_out = (unsigned)(p->ntiles < kMaxGrid ? p->ntiles : kMaxGrid);

@@ -9,11 +9,11 @@
 //   * a "group" is 64 consecutive elements of cuSZp block order: two 32-element
 //     blocks in 1D, one 8x8 (2D) or 4x4x4 (3D) block otherwise. One wave holds
 //     one group, one element per lane;
-//   * a workgroup is WAVES waves; each wave owns G consecutive groups, so a
-//     tile is WAVES*G groups. The grid is persistent: workgroup w handles
-//     tiles w, w+B, w+2B, ... and prefetches its next tile while it scans and
-//     stores the current one. No ticket atomics: one hot device-scope atomic
-//     serialises at about 12 ns per operation on MI355X;
+//   * a workgroup is WAVES waves and handles one tile: each wave owns kChunks
+//     chunks of G consecutive groups, so a tile is WAVES*kChunks*G groups.
+//     The grid has one workgroup per tile (tile = launch base + blockIdx.x).
+//     No ticket atomics: one hot device-scope atomic serialises at about
+//     12 ns per operation on MI355X;
 //   * lanes hold elements in "mirror" order: lane L holds element L^7, which is
 //     exactly the bit order of cuSZp's sign/plane bytes (element e at byte e/8,
 //     bit 7-e%8). Each lane packs its sign into bit 31 above the 31-bit
@@ -25,7 +25,8 @@
 //     wave with one lane per block;
 //   * the inter-tile exclusive scan of compressed bytes is a single-pass
 //     decoupled lookback: one 64-bit status word per tile packs
-//     {epoch, flag, value}, is read 256 predecessors at a time by one wave, and
+//     {epoch, flag, value}, is read 128 or 256 predecessors at a time by one
+//     wave, and
 //     needs only relaxed agent-scope atomics (no fences: the value travels in
 //     the same word as its flag). Forward progress does not rely on every
 //     workgroup being resident: a wave that has waited too long for a
@@ -62,6 +63,7 @@ constexpr uint32_t kErrOverflow = 1u;    // compressed stream exceeds output cap
 constexpr uint32_t kErrTruncated = 2u;   // stream shorter than its header describes
 constexpr uint32_t kErrStall = 4u;       // a lookback gave up waiting (internal error; output invalid)
 constexpr uint32_t kInfoStolen = 8u;     // a waiting wave computed a predecessor's byte count (not an error)
+constexpr uint32_t kWarnLossy = 16u;     // a 2D/3D outlier block has rate byte 32, which decoders read as rate 0
 constexpr uint32_t kErrMask = 7u;
 
 // Bound on lookback polls (each polls the status words once): far above any
@@ -87,8 +89,14 @@ struct Geom {
 struct ScanCtx {
     unsigned long long* status;     // tile status words, one per tile of the launch
     unsigned int* err;              // sticky error bits of the launch (kErrStall, kInfoStolen)
-    unsigned long long* diag;       // [0] first stalled tile + 1, [1] its missing predecessor, [2] word seen
+    unsigned long long* diag;       // [0] first stalled tile + 1, [1] its missing predecessor, [2] word seen;
+                                    // with `stats`: [4] polls that found a window unresolved, [5] lookbacks
+                                    // that polled, [6] predecessors claimed, [7] clock ticks spent waiting
     unsigned long long tag;         // epoch << kEpochShift
+    uint32_t steal_ticks;           // wait (s_memrealtime ticks) before a pending predecessor is claimed
+    uint32_t hold_every;            // scan stress: tile t with t % hold_every == hold_every - 1 waits
+    uint32_t hold_ticks;            //   hold_ticks before it publishes anything (0: off)
+    uint32_t stats;                 // nonzero: accumulate the lookback counters into diag[4..7]
 };
 
 // One array of a launch. The single-array kernels take it as an argument; the
@@ -329,14 +337,6 @@ __device__ __forceinline__ uint32_t load_u32u(const uint8_t* p) {
 __device__ __forceinline__ uint64_t ld_status(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// A possibly stale read (may be served by this XCD's L2). Status words only
-// move forward within a launch (nothing -> aggregate -> inclusive prefix; the
-// values never change) and carry the launch's epoch, so a stale word is still
-// a true statement about its tile; words that read as not ready are re-read
-// with ld_status.
-__device__ __forceinline__ uint64_t ld_status_cached(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 __device__ __forceinline__ void st_status(unsigned long long* p, uint64_t v) {
     __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -380,7 +380,8 @@ __device__ __forceinline__ bool status_ready(uint64_t w, uint64_t tag) {
 //
 // Tiles are numbered in launch order, so a predecessor normally belongs to a
 // workgroup that started earlier. Forward progress does not depend on that:
-// when a window stays unresolved for kStealTicks, the wave claims the farthest
+// when a window stays unresolved for sc.steal_ticks (kStealTicks unless a test
+// lowers it), the wave claims the farthest
 // pending predecessor of the window (CAS from the word it saw to CLAIM),
 // computes that tile's aggregate with agg_of(tile) and publishes it. A claim is
 // held only by a running wave that computes without waiting, so every wait
@@ -391,6 +392,12 @@ __device__ __forceinline__ uint64_t wave_lookback(const ScanCtx& sc, uint64_t ti
                                                   int lane, AggF&& agg_of) {
     constexpr int WIN = 64 * K;
     const uint64_t tag = sc.tag;
+    if (sc.hold_every && tile % sc.hold_every == sc.hold_every - 1) {
+        // scan stress (tests): this tile publishes late, so its successors
+        // claim it and compute its aggregate themselves; the wait is bounded
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        while (__builtin_amdgcn_s_memrealtime() - t0 < sc.hold_ticks) __builtin_amdgcn_s_sleep(8);
+    }
     if (tile == first) {
         if (lane == 0) st_status(sc.status + tile, tag | kFlagPrefix | agg);
         return 0;
@@ -408,6 +415,7 @@ __device__ __forceinline__ uint64_t wave_lookback(const ScanCtx& sc, uint64_t ti
             if (lane == 0) {
                 cas_status(sc.status + claimed, tag | kFlagClaim, tag | kFlagAgg | a);
                 atomicOr(sc.err, kInfoStolen);
+                if (sc.stats) atomicAdd(sc.diag + 6, 1ull);
             }
             claimed = -1;
         }
@@ -461,7 +469,7 @@ __device__ __forceinline__ uint64_t wave_lookback(const ScanCtx& sc, uint64_t ti
                 }
                 break;
             }
-            if (__builtin_amdgcn_s_memrealtime() - t_wait > kStealTicks &&
+            if (__builtin_amdgcn_s_memrealtime() - t_wait > sc.steal_ticks &&
                 !(((pend & kTagMask) == tag) && ((pend & kFlagMask) == kFlagClaim))) {
                 uint64_t seen = 0;
                 if (lane == 0) seen = cas_status(sc.status + pidx, pend, tag | kFlagClaim);
@@ -484,7 +492,14 @@ __device__ __forceinline__ uint64_t wave_lookback(const ScanCtx& sc, uint64_t ti
         if (P < WIN || stalled) break;
         hi -= WIN;
     }
-    if (lane == 0) st_status(sc.status + tile, tag | kFlagPrefix | (excl + agg));
+    if (lane == 0) {
+        st_status(sc.status + tile, tag | kFlagPrefix | (excl + agg));
+        if (sc.stats && spins) {
+            atomicAdd(sc.diag + 4, (unsigned long long)spins);
+            atomicAdd(sc.diag + 5, 1ull);
+            atomicAdd(sc.diag + 7, (unsigned long long)(__builtin_amdgcn_s_memrealtime() - t_wait));
+        }
+    }
     return excl;
 }
 
@@ -971,6 +986,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
     uint64_t excl = 0, wexcl = 0, ttot = 0;
     uint32_t wave_tot = 0;
     bool over = false;
+    bool lossy = false;   // a 2D/3D outlier block got rate byte 32 (see kWarnLossy)
     for (int pass = 0;; ++pass) {
         // pass 0: header bytes and staging; pass 1 (a wave that did not fit):
         // direct stores at gbase
@@ -1130,6 +1146,15 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             if constexpr (MODE == kOutlier) byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
             else byte = bitlen(bk1);
             const BlockView v = view_of<MODE, S>(byte);
+            if constexpr (MODE == kOutlier && DIM != 1) {
+                // R1 = 32 (a residual of magnitude 2^31, only from saturated or
+                // wrapped codes) is kept as the plain byte 32, which every
+                // outlier decoder reads as rate 0 (written by
+                // src/cuSZp_kernels_2D_f32.cu:1494-1505, masked with 0x1f by the
+                // decoder at :2320): the block decodes to zeros, so the call
+                // reports it
+                lossy |= ballot(blk && byte == 32u) != 0;
+            }
             const uint32_t sz = blk ? v.size : 0u;
             const uint32_t incl = scan_wave(sz);
             const uint32_t ctot = rdlane(incl, kWaveSize - 1);
@@ -1241,6 +1266,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
         if (lane == 0) {
             sm.wtot[wave] = wave_tot;
             sm.over[wave] = over;
+            if (lossy) atomicOr(cur.err, kWarnLossy);
         }
         __syncthreads();
 #pragma unroll

@@ -55,7 +55,7 @@ struct Lane {
     unsigned long long* d_status = nullptr;
     uint64_t cap_tiles = 0;
     uint64_t launches = 0;
-    unsigned long long* d_misc = nullptr;   // [0] epoch, [2] launch error bits, [4..6] diagnostics
+    unsigned long long* d_misc = nullptr;   // kMiscWords: [0] epoch, [2] launch bits, [4..7] diagnostics
     unsigned int* d_err = nullptr;
     unsigned long long* d_sizes = nullptr;
     unsigned int* h_err = nullptr;
@@ -144,16 +144,16 @@ int lane_init(Lane& L, size_t max_batch, uint64_t tiles) {
     SRV_HIP(hipEventCreateWithFlags(&L.ev, hipEventDisableTiming));
     SRV_HIP(hipMalloc((void**)&L.d_items, max_batch * sizeof(Item)));
     SRV_HIP(hipHostMalloc((void**)&L.h_items, max_batch * sizeof(Item), hipHostMallocDefault));
-    SRV_HIP(hipMalloc((void**)&L.d_misc, 8 * sizeof(unsigned long long)));
+    SRV_HIP(hipMalloc((void**)&L.d_misc, kMiscWords * sizeof(unsigned long long)));
     SRV_HIP(hipMalloc((void**)&L.d_err, max_batch * sizeof(unsigned int)));
     SRV_HIP(hipMalloc((void**)&L.d_sizes, max_batch * sizeof(unsigned long long)));
     SRV_HIP(hipHostMalloc((void**)&L.h_err, max_batch * sizeof(unsigned int), hipHostMallocDefault));
     SRV_HIP(hipHostMalloc((void**)&L.h_sizes, max_batch * sizeof(unsigned long long), hipHostMallocDefault));
-    SRV_HIP(hipHostMalloc((void**)&L.h_misc, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    SRV_HIP(hipHostMalloc((void**)&L.h_misc, kMiscWords * sizeof(unsigned long long), hipHostMallocDefault));
     SRV_HIP(hipMalloc((void**)&L.d_status, (kStatusGuard + tiles) * sizeof(unsigned long long)));
     L.cap_tiles = tiles;
     SRV_HIP(hipMemsetAsync(L.d_status, 0, (kStatusGuard + tiles) * sizeof(unsigned long long), L.st));
-    SRV_HIP(hipMemsetAsync(L.d_misc, 0, 8 * sizeof(unsigned long long), L.st));
+    SRV_HIP(hipMemsetAsync(L.d_misc, 0, kMiscWords * sizeof(unsigned long long), L.st));
     SRV_HIP(hipStreamSynchronize(L.st));
     return CUSZP_SUCCESS;
 }
@@ -201,9 +201,10 @@ int lane_launch(cuszp_server_s& s, Lane& L) {
     }
     const uint64_t need = tc > td ? tc : td;
     if (need > L.cap_tiles) {   // the lane is idle: its status array can be replaced
-        SRV_HIP(hipFree(L.d_status));
-        L.d_status = nullptr;
         uint64_t cap = L.cap_tiles * 2 > need ? L.cap_tiles * 2 : need;
+        if (L.d_status) SRV_HIP(hipFree(L.d_status));
+        L.d_status = nullptr;
+        L.cap_tiles = 0;   // a failed allocation below leaves no array to launch on
         SRV_HIP(hipMalloc((void**)&L.d_status, (kStatusGuard + cap) * sizeof(unsigned long long)));
         L.cap_tiles = cap;
         SRV_HIP(hipMemsetAsync(L.d_status, 0, (kStatusGuard + cap) * sizeof(unsigned long long), L.st));
@@ -215,12 +216,10 @@ int lane_launch(cuszp_server_s& s, Lane& L) {
     }
     SRV_HIP(hipMemcpyAsync(L.d_items, L.h_items, n * sizeof(Item), hipMemcpyHostToDevice, L.st));
     SRV_HIP(hipMemsetAsync(L.d_err, 0, n * sizeof(unsigned int), L.st));
-    SRV_HIP(hipMemsetAsync(L.d_misc + 2, 0, 6 * sizeof(unsigned long long), L.st));
-    ScanCtx sc;
-    sc.status = L.d_status + kStatusGuard;
-    sc.err = (unsigned int*)(L.d_misc + 2);
-    sc.diag = L.d_misc + 4;
-    sc.tag = 0;   // the kernels read the epoch word
+    SRV_HIP(hipMemsetAsync(L.d_misc + kMiscErr, 0, (kMiscCounters - kMiscErr) * sizeof(unsigned long long), L.st));
+    // the kernels read the epoch word for the tag
+    const ScanCtx sc = make_scan_ctx(L.d_status + kStatusGuard, (unsigned int*)(L.d_misc + kMiscErr),
+                                     L.d_misc + kMiscDiag, 0);
     const size_t nc = L.ncomp;
     if (nc) {
         SRV_HIP(launch_bump_epoch(L.d_misc, L.st));
@@ -234,14 +233,14 @@ int lane_launch(cuszp_server_s& s, Lane& L) {
     }
     SRV_HIP(hipMemcpyAsync(L.h_err, L.d_err, n * sizeof(unsigned int), hipMemcpyDeviceToHost, L.st));
     if (nc) SRV_HIP(hipMemcpyAsync(L.h_sizes, L.d_sizes, nc * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.st));
-    SRV_HIP(hipMemcpyAsync(L.h_misc, L.d_misc, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.st));
+    SRV_HIP(hipMemcpyAsync(L.h_misc, L.d_misc, kMiscWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, L.st));
     SRV_HIP(hipEventRecord(L.ev, L.st));
     L.busy = true;
     return CUSZP_SUCCESS;
 }
 
 void lane_retire(cuszp_server_s& s, Lane& L) {
-    const unsigned long long stall = L.h_misc[2] & kErrStall;
+    const unsigned long long stall = L.h_misc[kMiscErr] & kErrStall;
     for (size_t k = 0; k < L.reqs.size(); ++k) {
         const Req& r = L.reqs[k];
         cuszp_completion_t c;
@@ -253,7 +252,27 @@ void lane_retire(cuszp_server_s& s, Lane& L) {
         c.d_out = s.pool + r.off;
         c.bytes = r.dir == CUSZP_BATCH_COMPRESS ? (size_t)L.h_sizes[k]
                                                 : r.item.nbEle * (r.item.type == CUSZP_TYPE_FLOAT ? 4 : 8);
-        if (c.status) c.bytes = 0;
+        if (!status_delivers(c.status)) c.bytes = 0;
+        s.done.push_back(c);
+        s.held[r.id].complete = true;
+    }
+    L.reqs.clear();
+    L.ncomp = 0;
+    L.busy = false;
+}
+
+// Completes every request of lane L with status `rc` without running it (its
+// launch failed); the lane is left idle and empty.
+void lane_fail(cuszp_server_s& s, Lane& L, int rc) {
+    if (L.st) hipStreamSynchronize(L.st);   // whatever was enqueued before the failure has finished
+    for (const Req& r : L.reqs) {
+        cuszp_completion_t c;
+        std::memset(&c, 0, sizeof(c));
+        c.id = r.id;
+        c.direction = r.dir;
+        c.status = rc;
+        c.offset = r.off;
+        c.d_out = s.pool + r.off;
         s.done.push_back(c);
         s.held[r.id].complete = true;
     }
@@ -292,7 +311,13 @@ int progress(cuszp_server_s& s) {
         L.ncomp = L.reqs.size();
         L.reqs.insert(L.reqs.end(), dec.begin(), dec.end());
         const int rc = lane_launch(s, L);
-        if (rc) return rc;
+        if (rc) {
+            // the batch did not start: its requests complete with the error
+            // (their outputs stay held until released) and the lane is idle
+            // and empty again
+            lane_fail(s, L, rc);
+            return rc;
+        }
     }
     return CUSZP_SUCCESS;
 }

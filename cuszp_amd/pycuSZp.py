@@ -13,9 +13,10 @@ the absolute one exactly as the reference CLI does (examples/cuSZp.cpp:176-210).
 from __future__ import annotations
 
 import ctypes
+import warnings
 from typing import Optional, Tuple
 
-from ._lib import Uint3, load
+from ._lib import SCAN_INFO_STOLEN, WARNING_BOUND_EXCEEDED, ScanReport, Uint3, load
 
 lib = load()
 
@@ -23,6 +24,11 @@ CUSZP_DIM_1D, CUSZP_DIM_2D, CUSZP_DIM_3D = 1, 2, 3
 CUSZP_MODE_FIXED, CUSZP_MODE_PLAIN, CUSZP_MODE_OUTLIER = 0, 1, 2
 CUSZP_TYPE_FLOAT, CUSZP_TYPE_DOUBLE = 0, 1
 MODES = {"fixed": 0, "plain": 1, "outlier": 2}
+
+
+class BoundExceededWarning(UserWarning):
+    """The stream is complete, but a 2D/3D outlier-mode block holds a residual
+    of magnitude 2^31 and decodes to zeros (status CUSZP_WARNING_BOUND_EXCEEDED)."""
 
 
 class CuSZpError(RuntimeError):
@@ -105,9 +111,35 @@ def compress_ex(d_oriData, d_cmpBytes, capacity: int, num_elements: int, error_b
     size = ctypes.c_size_t(0)
     rc = lib.cuSZp_compress_ex(_ptr(d_oriData), _ptr(d_cmpBytes), capacity, num_elements, ctypes.byref(size),
                                float(error_bound), dim, Uint3(*dims), data_type, mode, _stream(stream))
-    if rc:
+    if rc == WARNING_BOUND_EXCEEDED:
+        warnings.warn(lib.cuSZp_status_string(rc).decode(), BoundExceededWarning, stacklevel=2)
+    elif rc:
         raise CuSZpError(rc, "compress")
     return size.value
+
+
+def scan_report(stream=None) -> dict:
+    """Scan diagnostics of the single-array entries on `stream` since the last
+    report (cuSZp_scan_report): info bits and, while the stats switch is on,
+    lookback polls, waits, claims and wait time in microseconds."""
+    r = ScanReport()
+    rc = lib.cuSZp_scan_report(_stream(stream), ctypes.byref(r))
+    if rc:
+        raise CuSZpError(rc, "scan report")
+    return _report_dict(r)
+
+
+def _report_dict(r) -> dict:
+    return {"stolen": bool(r.info & SCAN_INFO_STOLEN), "info": r.info, "polls": r.polls, "waits": r.waits,
+            "claims": r.claims, "wait_us": r.wait_ticks / 100.0}
+
+
+def scan_config(steal_ticks: int = -1, hold_every: int = 0, hold_ticks: int = 0, stats: bool = False) -> None:
+    """Scan settings of later launches (cuSZp_scan_config); the defaults restore
+    normal operation."""
+    rc = lib.cuSZp_scan_config(int(steal_ticks), int(hold_every), int(hold_ticks), 1 if stats else 0)
+    if rc:
+        raise CuSZpError(rc, "scan config")
 
 
 def decompress_ex(d_decData, d_cmpBytes, num_elements: int, compressed_size: int, error_bound: float,
@@ -290,6 +322,13 @@ class BatchPlan:
 
     def device_sizes(self) -> int:
         return lib.cuSZp_batch_plan_device_sizes(self._plan) or 0
+
+    def scan_report(self, stream=None) -> dict:
+        r = ScanReport()
+        rc = lib.cuSZp_batch_plan_scan_report(self._plan, _stream(stream), ctypes.byref(r))
+        if rc:
+            raise CuSZpError(rc, "batch scan report")
+        return _report_dict(r)
 
     def close(self) -> None:
         if self._plan:

@@ -65,7 +65,13 @@ typedef enum {
     CUSZP_ERROR_OUTPUT_TOO_SMALL = 4,   /* compressed stream exceeds cmpCapacity */
     CUSZP_ERROR_CORRUPT_STREAM = 5,     /* cmpSize shorter than the header describes */
     CUSZP_ERROR_HIP = 6,                /* a HIP runtime call failed */
-    CUSZP_ERROR_INTERNAL = 7            /* device-side scan watchdog fired; output invalid */
+    CUSZP_ERROR_INTERNAL = 7,           /* device-side scan watchdog fired; output invalid */
+    /* Not an error: the stream is written and *cmpSize is valid, but a 2D/3D
+     * outlier-mode block holds a residual of magnitude 2^31 (saturated or
+     * wrapped quantisation codes). Its rate byte 32 is read as rate 0 by every
+     * outlier decoder, the reference's included (src/cuSZp_kernels_2D_f32.cu:2320),
+     * so that block decompresses to zeros, beyond the error bound. */
+    CUSZP_WARNING_BOUND_EXCEEDED = 8
 } cuszp_status_t;
 
 /* ---- generic entries: include/cuSZp.h:38-45 (reference) ---------------- */
@@ -159,10 +165,41 @@ int cuSZp_last_async_status(hipStream_t stream);
 
 const char* cuSZp_status_string(int status);
 
+/* ---- scan diagnostics (no reference counterpart) ----
+ * The inter-tile scan is a single-pass decoupled lookback. A tile that waits
+ * too long for a predecessor claims it and computes that tile's byte count
+ * itself (forward progress without assuming that every workgroup is
+ * resident); CUSZP_SCAN_INFO_STOLEN records that this happened. It is
+ * information, not an error: the output is the same either way. */
+#define CUSZP_SCAN_INFO_STOLEN 8u
+typedef struct {
+    unsigned int info;               /* CUSZP_SCAN_INFO_* bits seen since the last report */
+    unsigned long long polls;        /* lookback polls that found their window unresolved (*) */
+    unsigned long long waits;        /* lookbacks that polled at least once (*) */
+    unsigned long long claims;       /* predecessors whose byte count a waiting tile computed (*) */
+    unsigned long long wait_ticks;   /* 100 MHz clock ticks spent in those waits (*) */
+} cuszp_scan_report_t;               /* (*) counted only while the stats switch is on */
+/* What the single-array entries on `stream` (this device) reported since the
+ * last call; clears it. Synchronises `stream`. */
+int cuSZp_scan_report(hipStream_t stream, cuszp_scan_report_t* report);
+/* Scan settings for launches made after the call (process-wide; a captured
+ * batch graph keeps the settings of its capture):
+ *   steal_ticks  wait before claiming a pending predecessor, in 100 MHz clock
+ *                ticks (negative: the default, 10,000 = 100 us; 0: claim at
+ *                the first unresolved poll);
+ *   hold_every   test stress: tile t with t % hold_every == hold_every - 1
+ *   hold_ticks     publishes only after waiting hold_ticks (0: off; at most
+ *                  10,000,000 = 100 ms, else CUSZP_ERROR_INVALID_ARGUMENT), so
+ *                  that its successors claim it;
+ *   stats        nonzero: count polls, waits, claims and wait time (costs one
+ *                device atomic per waiting tile). */
+int cuSZp_scan_config(int steal_ticks, unsigned int hold_every, unsigned int hold_ticks, int stats);
+
 /* ---- batched form (no reference counterpart) ----
  * Many arrays of any sizes, dtypes, modes, dims and error bounds compress (or
- * decompress) in ONE persistent kernel launch. Every array's stream is
- * byte-identical to its single-array stream. A plan validates the arrays once,
+ * decompress) in ONE kernel launch with one workgroup per tile of the
+ * concatenated arrays. Every array's stream is byte-identical to its
+ * single-array stream. A plan validates the arrays once,
  * uploads their descriptors, owns the scan state and can hold the launch as a
  * captured hipGraph, so execute() is a single graph launch. */
 typedef struct {
@@ -194,7 +231,8 @@ int cuSZp_batch_plan_create(cuszp_batch_plan_t* plan, const cuszp_batch_item_t* 
 int cuSZp_batch_plan_execute(cuszp_batch_plan_t plan, hipStream_t stream);
 /* Synchronises `stream`; copies each array's stream length (compress) and
  * status to the host arrays (either may be null); returns the first non-zero
- * status or CUSZP_SUCCESS. */
+ * status or CUSZP_SUCCESS. An array with CUSZP_WARNING_BOUND_EXCEEDED keeps
+ * its stream length. */
 int cuSZp_batch_plan_results(cuszp_batch_plan_t plan, size_t* cmpSizes, int* statuses, hipStream_t stream);
 /* Device array of per-array stream lengths written by each execute (compress). */
 const unsigned long long* cuSZp_batch_plan_device_sizes(cuszp_batch_plan_t plan);
@@ -205,6 +243,8 @@ const unsigned long long* cuSZp_batch_plan_device_sizes(cuszp_batch_plan_t plan)
  * *waves and the workgroups of its (first) launch, one per tile, in *grid. */
 int cuSZp_batch_plan_tune(cuszp_batch_plan_t plan, hipStream_t stream, int* waves, unsigned* grid);
 void cuSZp_batch_plan_destroy(cuszp_batch_plan_t plan);
+/* cuSZp_scan_report for the plan's launches (synchronises `stream`). */
+int cuSZp_batch_plan_scan_report(cuszp_batch_plan_t plan, hipStream_t stream, cuszp_scan_report_t* report);
 
 /* One-shot batched calls: create a plan, execute, read results, destroy. */
 int cuSZp_compress_batch(const cuszp_batch_item_t* items, size_t count, size_t* cmpSizes, int* statuses,

@@ -89,11 +89,26 @@ def test_residual_int_min(device, oracle, dim, mode, dtype):
     x[20] = 5.0
     x = x.astype(dtype)
     ref = oracle.compress(x, 0.5, mode, dim, dims)
-    got = gpu_compress(x, 0.5, mode, dim, dims)
+    import warnings
+
+    from cuszp_amd.pycuSZp import BoundExceededWarning
+
+    with warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter("always")
+        got = gpu_compress(x, 0.5, mode, dim, dims)
     assert got == ref
+    lossy = [w for w in caught if issubclass(w.category, BoundExceededWarning)]
+    # outlier mode keeps R1 = 32 as the plain byte 32, which decoders read as
+    # rate 0 (the block decodes to zeros): the call warns; fixed and plain
+    # mode encode all 32 planes and decode them exactly
+    assert len(lossy) == (1 if mode == 2 else 0)
+    if mode == 2:
+        assert ref[0] == 32   # header byte of the block: plain rate 32
     dref = oracle.decompress(ref, n, dtype, 0.5, mode, dim, dims)
     dgot = gpu_decompress(ref, n, dtype, 0.5, mode, dim, dims)
     assert np.array_equal(dgot.view(np.uint8), dref.view(np.uint8))
+    if mode == 2:
+        assert not dgot.any()   # the whole block decoded as zeros
 
 
 @pytest.mark.parametrize("eb", [1e-1, 1e-3, 1e-6])
