@@ -12,12 +12,13 @@ on torch's current stream. The JSON line reports:
   cpu_baseline     the C++ oracle on the host cores, on a bounded sample
 
 Workloads (--workload):
-  3d_outlier_f32   (default) 512^3 f32 Gaussian random field, k^-11/3 spectrum,
+  1d_plain_f32_4g  (default) 2^30 f32 sinusoids + noise (4 GiB), 1D plain, rel 1e-3:
+                   the large-array roofline run, both directions timed separately
+  3d_outlier_f32   512^3 f32 Gaussian random field, k^-11/3 spectrum,
                    [-50,50] + 0.1 % spikes, 3D outlier mode, rel eb 1e-4
   2d_outlier_f32   same volume as 512 independent 512x512 slices (2D)
   1d_outlier_f32   same volume, 1D
   1d_plain_f32     16,777,216 f32 sum of 8 sinusoids + noise, 20 % zero runs, plain, rel 1e-3
-  1d_plain_f32_4g  2^30 f32 sinusoids + noise (4 GiB), plain, rel 1e-3
   2d_fixed_f64     8192x8192 f64 N(0, 0.02), 2D fixed mode, abs eb 1e-3
   batch_1000       1,000 arrays of log-uniform sizes 1,024..16,777,216, 70 % f32 /
                    30 % f64: half dense-layer weights (fixed mode, abs eb 1e-3),
@@ -143,7 +144,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="3d_outlier_f32", choices=sorted(WORKLOADS) + ["batch_1000", "serve_20000"])
+    ap.add_argument("--workload", default="1d_plain_f32_4g", choices=sorted(WORKLOADS) + ["batch_1000", "serve_20000"])
     ap.add_argument("--cpu-sample-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -578,7 +579,7 @@ def cpu_baseline_batch(np, arrs, budget_s):
         done += 1
         if tc + td > budget_s:
             break
-    return {
+    out = {
         "value": round(2 * raw / (tc + td) / 1e9, 4),
         "unit": "GB/s",
         "cores": threads,
@@ -587,6 +588,8 @@ def cpu_baseline_batch(np, arrs, budget_s):
         "compress_GBps": round(raw / tc / 1e9, 4),
         "decompress_GBps": round(raw / td / 1e9, 4),
     }
+    out.update(host_cpu())
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -790,13 +793,15 @@ def cpu_baseline_serve(np, reqs, trace, slice_of, store, eb_of, budget_s):
         done += 1
         if t_all > budget_s:
             break
-    return {
+    out = {
         "value": round(raw / t_all / 1e9, 4),
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
         "sample": f"the trace's first {done} compress requests ({raw} bytes), compress only",
     }
+    out.update(host_cpu())
+    return out
 
 
 def copy_calibration(torch, steps):
@@ -815,8 +820,32 @@ def copy_calibration(torch, steps):
                       "copy_GBps": round(2 * a.numel() * 4 / dt / 1e9, 2)}), flush=True)
 
 
+def host_cpu():
+    """CPU model, logical CPUs and physical cores of this host (/proc/cpuinfo)."""
+    model, cores, logical = None, set(), 0
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "processor":
+                logical += 1
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_logical_cpus": logical or os.cpu_count(),
+            "host_physical_cores": len(cores) or None}
+
+
 def cpu_baseline(np, x, wl, eb, s, budget_s):
-    """Oracle (C++/OpenMP, all host threads) on a leading slice of the same array."""
+    """Oracle (C++/OpenMP) on a leading slice of the same array: with the
+    process's OpenMP threads (`cores`), and with one thread on a shorter run."""
     from oracle import oracle as O
 
     threads = O.max_threads()
@@ -856,15 +885,34 @@ def cpu_baseline(np, x, wl, eb, s, budget_s):
         if time.perf_counter() - t_all > budget_s or reps >= 50:
             break
     raw = m * s
-    return {
+    # one thread: the same slice, round trips for about a fifth of the budget
+    tc1 = td1 = 0.0
+    reps1 = 0
+    t_all = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        c = O.compress(h, eb, wl["mode"], wl["dim"], dims, nthreads=1)
+        t1 = time.perf_counter()
+        O.decompress(c, m, h.dtype, eb, wl["mode"], wl["dim"], dims, nthreads=1)
+        tc1 += t1 - t0
+        td1 += time.perf_counter() - t1
+        reps1 += 1
+        if time.perf_counter() - t_all > budget_s / 5 or reps1 >= 10:
+            break
+    out = {
         "value": round(2 * raw * reps / (tc + td) / 1e9, 4),
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample}, {reps} compress+decompress round trips",
+        "sample": f"{sample}, {reps} compress+decompress round trips ({reps1} with one thread)",
         "compress_GBps": round(raw * reps / tc / 1e9, 4),
         "decompress_GBps": round(raw * reps / td / 1e9, 4),
+        "one_thread_GBps": round(2 * raw * reps1 / (tc1 + td1) / 1e9, 4),
+        "one_thread_compress_GBps": round(raw * reps1 / tc1 / 1e9, 4),
+        "one_thread_decompress_GBps": round(raw * reps1 / td1 / 1e9, 4),
     }
+    out.update(host_cpu())
+    return out
 
 
 if __name__ == "__main__":

@@ -259,7 +259,7 @@ def decompress_tensor(cmp, size: int, num_elements: int, dtype, error_bound: flo
 
 
 # ---------------------------------------------------------------------------
-# batched form: many arrays in one persistent launch (include/cuSZp.h,
+# batched form: many arrays in one launch per direction (include/cuSZp.h,
 # cuSZp_batch_plan_*). Every array's stream is byte-identical to its
 # single-array stream.
 # ---------------------------------------------------------------------------

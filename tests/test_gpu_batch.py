@@ -1,5 +1,5 @@
 """Batched form: many arrays of mixed size, dtype, dim, mode and error bound in
-one persistent launch (cuSZp_batch_plan_*). Every array's stream must be
+one launch per direction (cuSZp_batch_plan_*). Every array's stream must be
 byte-identical to its single-array stream (which the parity tests tie to the
 oracle), and the batched decompression bit-identical to the single-array one;
 a graph-captured plan must give the same bytes on every replay, also with other

@@ -139,3 +139,80 @@ def test_2d_partial_tile_edges(oracle):
     exp = hexb("03000000") + bytes(8) + hexb(planes)
     y = check(oracle, x, 0.5, 1, 2, (4, 3, 1), exp)
     assert np.array_equal(y, x)
+
+
+def test_2d_fixed_tile(oracle):
+    # fixed mode: d = q. q(0,0) = -3, q(1,2) = 5, q(7,7) = 2 -> rate 3 (|5| = 0b101).
+    # Byte k of the sign field and of each plane is row k; column c is bit 7-c.
+    x = np.zeros((8, 8), np.float32)
+    x[0, 0], x[1, 2], x[7, 7] = -3, 5, 2
+    sign = "8000000000000000"     # (0,0) negative
+    p0 = "8020000000000000"       # 3 (0,0) and 5 (1,2) have bit 0
+    p1 = "8000000000000001"       # 3 (0,0) and 2 (7,7) have bit 1
+    p2 = "0020000000000000"       # 5 (1,2) has bit 2
+    exp = hexb("03000000") + hexb(sign + p0 + p1 + p2)
+    y = check(oracle, x.ravel(), 0.5, 0, 2, (8, 8, 1), exp)
+    assert np.array_equal(y, x.ravel())
+
+
+def test_2d_outlier_selected_two_prefix_bytes(oracle):
+    # outlier mode, one 8x8 tile of 1000 with x(0,1) = 1001:
+    # d(0,0) = 1000, d(0,1) = 1, d(0,2) = -1, every other residual 0.
+    # R1 = bitlen(1000) = 10 -> 8 + 8*10 = 88 bytes; R2 = 1, nb = ceil(10/8) = 2
+    # -> 2 + 8 + 8 = 18 bytes: selected, byte 0x80 | (2-1)<<5 | 1 = 0xA1.
+    # Payload: |d(0,0)| little-endian (e8 03), sign (e=2: byte 0 bit 5),
+    # plane 0 with element 0 cleared (e=1, e=2: bits 6 and 5 of byte 0).
+    x = np.full((8, 8), 1000, np.float32)
+    x[0, 1] = 1001
+    exp = hexb("a1000000") + hexb("e803" + "2000000000000000" + "6000000000000000")
+    y = check(oracle, x.ravel(), 0.5, 2, 2, (8, 8, 1), exp)
+    assert np.array_equal(y, x.ravel())
+
+
+def test_2d_plain_second_tile_only(oracle):
+    # 16 x 8 slice = two tiles along x (b = by*Bx + bx); tile 0 is all zero (rate 0,
+    # no payload); tile 1 has x(0,8) = 1, its predictor restarts at the tile:
+    # d(0,0) = 1, d(0,1) = -1, d(1,0) = -1 -> rate 1.
+    x = np.zeros((8, 16), np.float32)
+    x[0, 8] = 1
+    exp = hexb("00010000") + hexb("4080000000000000" + "c080000000000000")
+    y = check(oracle, x.ravel(), 0.5, 1, 2, (16, 8, 1), exp)
+    assert np.array_equal(y, x.ravel())
+
+
+def test_3d_fixed_block(oracle):
+    # fixed mode, 4x4x4 block, e = 16z + 4y + x: q(0,0,0) = 6, q(1,2,3) = -1 (e = 27),
+    # q(3,3,3) = 4 (e = 63) -> rate 3. e = 27 is byte 3, bit 4; e = 63 byte 7, bit 0.
+    x = np.zeros((4, 4, 4), np.float32)
+    x[0, 0, 0], x[1, 2, 3], x[3, 3, 3] = 6, -1, 4
+    sign = "0000001000000000"
+    p0 = "0000001000000000"       # 1 (e=27)
+    p1 = "8000000000000000"       # 6 (e=0)
+    p2 = "8000000000000001"       # 6 (e=0), 4 (e=63)
+    exp = hexb("03000000") + hexb(sign + p0 + p1 + p2)
+    y = check(oracle, x.ravel(), 0.5, 0, 3, (4, 4, 4), exp)
+    assert np.array_equal(y, x.ravel())
+
+
+def test_3d_outlier_selected(oracle):
+    # outlier mode, a 4x4x4 block of 300 with x(0,0,1) = 302:
+    # d(0,0,0) = 300, d(0,0,1) = 2, d(0,0,2) = -2, others 0.
+    # R1 = bitlen(300) = 9 -> 8 + 72 = 80 bytes; R2 = 2, nb = 2 -> 2 + 8 + 16 = 26:
+    # selected, byte 0x80 | 1<<5 | 2 = 0xA2. Prefix 300 = 2c 01; sign e=2 (byte 0
+    # bit 5); plane 0 empty; plane 1: e=1, e=2 (0x60).
+    x = np.full((4, 4, 4), 300, np.float32)
+    x[0, 0, 1] = 302
+    exp = hexb("a2000000") + hexb("2c01" + "2000000000000000" + "0000000000000000" + "6000000000000000")
+    y = check(oracle, x.ravel(), 0.5, 2, 3, (4, 4, 4), exp)
+    assert np.array_equal(y, x.ravel())
+
+
+def test_3d_outlier_not_selected_keeps_plain_bytes(oracle):
+    # outlier mode where the plain encoding is smaller: d(0,0,0) = 3, one residual
+    # 2 -> R1 = 2: 8 + 16 = 24 bytes vs R2 = 2, nb = 1: 1 + 8 + 16 = 25 -> plain,
+    # byte 2, and element 0 stays in the planes.
+    x = np.full((4, 4, 4), 3, np.float32)
+    x[3, 3, 3] = 5   # d(3,3,3) = 5 - 3 = 2 (e = 63: byte 7, bit 0)
+    exp = hexb("02000000") + hexb("0000000000000000" + "8000000000000000" + "8000000000000001")
+    y = check(oracle, x.ravel(), 0.5, 2, 3, (4, 4, 4), exp)
+    assert np.array_equal(y, x.ravel())
