@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--workload", default="1d_plain_f32_4g", choices=sorted(WORKLOADS) + ["batch_1000", "serve_20000"])
     ap.add_argument("--cpu-sample-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scan-stats", action="store_true",
+                    help="instead of timing, run --steps compress + decompress rounds with the lookback "
+                         "counters on and print them (polls, waits, claims, wait time) per direction")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -216,6 +219,10 @@ def main():
     fin = torch.isfinite(x) & (x.abs() * (0.5 / ebT) < 2.0**30)
     max_err = ((x.double() - y.double()).abs()[fin].max() / ebT).item()
     log(f"[bench] C={C} CR={n * s / C:.3f} max_err/eb={max_err:.7f}")
+
+    if args.scan_stats:
+        scan_stats(args, L, torch, stream, sp, xp, cp, yp, dsp, cap, n, C, eb, wl, U, typ)
+        return
 
     def step(evs=None):
         if evs is not None:
@@ -802,6 +809,53 @@ def cpu_baseline_serve(np, reqs, trace, slice_of, store, eb_of, budget_s):
     }
     out.update(host_cpu())
     return out
+
+
+def scan_stats(args, L, torch, stream, sp, xp, cp, yp, dsp, cap, n, C, eb, wl, U, typ):
+    """Lookback counters of the single-array kernels on this workload
+    (cuSZp_scan_config stats switch: one device atomic per waiting tile):
+    per direction, the tiles that polled an unresolved window, the polls, the
+    predecessors claimed and the mean wait, over --steps launches."""
+    from cuszp_amd import pycuSZp as P
+
+    P.scan_config(stats=True)
+    P.scan_report(stream)   # start from zero
+    out = {"workload": args.workload, "launches": args.steps}
+    try:
+        for name in ("compress", "decompress"):
+            for _ in range(args.steps):
+                if name == "compress":
+                    rc = L.cuSZp_compress_async(xp, cp, cap, n, dsp, eb, wl["dim"], U, typ, wl["mode"], sp)
+                else:
+                    rc = L.cuSZp_decompress_async(yp, cp, n, C, eb, wl["dim"], U, typ, wl["mode"], sp)
+                assert rc == 0
+            torch.cuda.synchronize()
+            r = P.scan_report(stream)
+            out[name] = {"waiting_tiles_per_launch": r["waits"] / args.steps,
+                         "polls_per_launch": r["polls"] / args.steps,
+                         "claims_per_launch": r["claims"] / args.steps,
+                         "mean_wait_us": round(r["wait_us"] / max(r["waits"], 1), 3),
+                         "wait_us_per_launch": round(r["wait_us"] / args.steps, 2)}
+    finally:
+        P.scan_config()
+    out["tiles_per_launch"] = tile_count(n, wl["dim"], wl.get("dims", (0, 0, 0)), typ)
+    print(json.dumps(out), flush=True)
+
+
+def tile_count(n, dim, dims, typ):
+    """Tiles (workgroups) of a single-array launch: 64-value groups, tiles of
+    8 waves x 8 chunks x G groups (G = 8 f32, 4 f64), 4 waves below 128 tiles."""
+    if dim == 1:
+        groups = -(-n // 32768) * 512
+    else:
+        x, y, z = dims
+        if dim == 2:
+            groups = -(-x // 8) * -(-y // 8) * z
+        else:
+            groups = -(-x // 4) * -(-y // 4) * -(-z // 4)
+    g = 4 if typ else 8
+    waves = 8 if -(-groups // (8 * 8 * g)) >= 128 else 4
+    return -(-groups // (waves * 8 * g))
 
 
 def copy_calibration(torch, steps):

@@ -37,6 +37,7 @@ EXPORTED = (
         "cuSZp_last_async_status",
         "cuSZp_scan_report",
         "cuSZp_scan_config",
+        "cuSZp_debug_fail_allocs",
         "cuSZp_batch_plan_scan_report",
         "cuSZp_status_string",
         "cuSZp_version",
@@ -59,6 +60,10 @@ EXPORTED = (
         "cuSZp_rel_error_bound",
         "cuSZp_quality",
         "cuSZp_ssim",
+        "cuSZp_container_write_header",
+        "cuSZp_container_read_header",
+        "cuSZp_compress_container",
+        "cuSZp_decompress_container",
     ]
 )
 
@@ -91,6 +96,25 @@ class ScanReport(ctypes.Structure):
 
 SCAN_INFO_STOLEN = 8
 WARNING_BOUND_EXCEEDED = 8
+
+
+class ContainerInfo(ctypes.Structure):
+    """cuszp_container_info_t (include/cuSZp.h, self-describing container)."""
+
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("type", ctypes.c_int),
+        ("mode", ctypes.c_int),
+        ("relative", ctypes.c_int),
+        ("dims", Uint3),
+        ("nbEle", ctypes.c_ulonglong),
+        ("errorBound", ctypes.c_double),
+        ("relBound", ctypes.c_double),
+        ("streamBytes", ctypes.c_ulonglong),
+    ]
+
+
+CONTAINER_HEADER_BYTES = 64
 
 
 class Completion(ctypes.Structure):
@@ -160,6 +184,8 @@ def load() -> ctypes.CDLL:
     L.cuSZp_scan_report.restype = i32
     L.cuSZp_scan_config.argtypes = [i32, ctypes.c_uint, ctypes.c_uint, i32]
     L.cuSZp_scan_config.restype = i32
+    L.cuSZp_debug_fail_allocs.argtypes = [ctypes.c_uint]
+    L.cuSZp_debug_fail_allocs.restype = i32
     L.cuSZp_batch_plan_scan_report.argtypes = [vp, vp, ctypes.POINTER(ScanReport)]
     L.cuSZp_batch_plan_scan_report.restype = i32
     L.cuSZp_status_string.argtypes = [i32]
@@ -198,6 +224,15 @@ def load() -> ctypes.CDLL:
     L.cuSZp_server_destroy.restype = None
     L.cuSZp_server_replay.argtypes = [vp, ctypes.POINTER(Request), sz, i32, pd, ctypes.POINTER(i32), pd]
     L.cuSZp_server_replay.restype = i32
+    pci = ctypes.POINTER(ContainerInfo)
+    L.cuSZp_container_write_header.argtypes = [pci, vp]
+    L.cuSZp_container_write_header.restype = i32
+    L.cuSZp_container_read_header.argtypes = [vp, sz, pci]
+    L.cuSZp_container_read_header.restype = i32
+    L.cuSZp_compress_container.argtypes = [vp, vp, sz, sz, psz, f64, f64, i32, Uint3, i32, i32, vp]
+    L.cuSZp_compress_container.restype = i32
+    L.cuSZp_decompress_container.argtypes = [vp, sz, vp, sz, pci, vp]
+    L.cuSZp_decompress_container.restype = i32
     L.cuSZp_value_range.argtypes = [vp, sz, i32, ctypes.POINTER(f64), ctypes.POINTER(f64), vp]
     L.cuSZp_value_range.restype = i32
     L.cuSZp_rel_error_bound.argtypes = [vp, sz, i32, f32, ctypes.POINTER(f32), vp]

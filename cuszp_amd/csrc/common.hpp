@@ -39,6 +39,26 @@ inline ScanSettings& scan_settings() {
     return s;
 }
 
+// Allocation of a scan-state status array (the arrays that grow with the
+// work: single-array slots, batch plans, serving lanes). A test can make the
+// next `n` of these fail as if the device were out of memory
+// (cuSZp_debug_fail_allocs), to check that a failed growth leaves the state
+// usable by the next call.
+inline std::atomic<uint32_t>& failing_allocs() {
+    static std::atomic<uint32_t> n{0};
+    return n;
+}
+inline hipError_t scan_state_alloc(unsigned long long** p, size_t bytes) {
+    uint32_t n = failing_allocs().load(std::memory_order_relaxed);
+    while (n && !failing_allocs().compare_exchange_weak(n, n - 1, std::memory_order_relaxed)) {
+    }
+    if (n) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc((void**)p, bytes);
+}
+
 // Scan context of a launch over `status` (kStatusGuard guard words in front
 // are the caller's), with the launch's error word and the diagnostic words
 // diag[0..7].

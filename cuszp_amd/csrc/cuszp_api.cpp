@@ -85,7 +85,7 @@ int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
         // kStatusGuard words in front of the status words (lookback windows of
         // the first tiles read there)
         unsigned long long* d = nullptr;
-        CUSZP_HIP(hipMalloc((void**)&d, (kStatusGuard + cap) * sizeof(unsigned long long)));
+        CUSZP_HIP(scan_state_alloc(&d, (kStatusGuard + cap) * sizeof(unsigned long long)));
         s.d_status = d;
         s.cap_tiles = cap;
         CUSZP_HIP(hipMemsetAsync(s.d_status, 0, (kStatusGuard + cap) * sizeof(unsigned long long), st));
@@ -306,6 +306,11 @@ int cuSZp_scan_report(hipStream_t stream, cuszp_scan_report_t* report) {
     s.info = 0;
     CUSZP_HIP(hipMemsetAsync(s.d_misc + kMiscCounters, 0, 4 * sizeof(unsigned long long), stream));
     CUSZP_HIP(hipStreamSynchronize(stream));
+    return CUSZP_SUCCESS;
+}
+
+int cuSZp_debug_fail_allocs(unsigned int count) {
+    failing_allocs().store(count, std::memory_order_relaxed);
     return CUSZP_SUCCESS;
 }
 

@@ -104,7 +104,7 @@ int build_table(cuszp_batch_plan_s& p, hipStream_t st) {
         p.d_status = nullptr;
         p.cap_tiles = 0;
         // kStatusGuard words in front (lookback windows of first tiles read there)
-        BATCH_HIP(hipMalloc((void**)&p.d_status, (kStatusGuard + need) * sizeof(unsigned long long)));
+        BATCH_HIP(scan_state_alloc(&p.d_status, (kStatusGuard + need) * sizeof(unsigned long long)));
         p.cap_tiles = need;
     }
     // stale status words must not carry the current epoch: start from zero and

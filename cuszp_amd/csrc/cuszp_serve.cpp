@@ -205,7 +205,7 @@ int lane_launch(cuszp_server_s& s, Lane& L) {
         if (L.d_status) SRV_HIP(hipFree(L.d_status));
         L.d_status = nullptr;
         L.cap_tiles = 0;   // a failed allocation below leaves no array to launch on
-        SRV_HIP(hipMalloc((void**)&L.d_status, (kStatusGuard + cap) * sizeof(unsigned long long)));
+        SRV_HIP(scan_state_alloc(&L.d_status, (kStatusGuard + cap) * sizeof(unsigned long long)));
         L.cap_tiles = cap;
         SRV_HIP(hipMemsetAsync(L.d_status, 0, (kStatusGuard + cap) * sizeof(unsigned long long), L.st));
         L.launches = 0;

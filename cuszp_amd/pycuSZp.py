@@ -259,6 +259,77 @@ def decompress_tensor(cmp, size: int, num_elements: int, dtype, error_bound: flo
 
 
 # ---------------------------------------------------------------------------
+# self-describing container: a 64-byte header (type, mode, dims, bound,
+# length; layout in include/cuSZp.h) in front of the unchanged raw stream
+# ---------------------------------------------------------------------------
+from ._lib import CONTAINER_HEADER_BYTES, ContainerInfo  # noqa: E402
+
+
+def _info_dict(ci) -> dict:
+    return {"dim": ci.dim, "type": ci.type, "mode": ci.mode, "relative": bool(ci.relative),
+            "dims": (ci.dims.x, ci.dims.y, ci.dims.z), "num_elements": ci.nbEle, "error_bound": ci.errorBound,
+            "rel_bound": ci.relBound, "stream_bytes": ci.streamBytes}
+
+
+def read_container_header(data: bytes) -> dict:
+    """Decodes and validates the header at the start of `data` (host bytes)."""
+    ci = ContainerInfo()
+    buf = (ctypes.c_ubyte * max(len(data), 1)).from_buffer_copy(data if data else b"\0")
+    rc = lib.cuSZp_container_read_header(buf, len(data), ctypes.byref(ci))
+    if rc:
+        raise CuSZpError(rc, "container header")
+    return _info_dict(ci)
+
+
+def compress_container(t, error_bound: float, mode="plain", dim: int = 1,
+                       dims: Optional[Tuple[int, int, int]] = None, rel_bound: float = 0.0, stream=None):
+    """Compress a contiguous float32/float64 CUDA tensor into a container.
+    Returns (uint8 tensor, container bytes); the raw stream starts at byte 64.
+    With rel_bound > 0 the header records it (error_bound is then the absolute
+    bound derived from it, e.g. rel_to_abs(t, rel_bound))."""
+    import torch
+
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    m = MODES[mode] if isinstance(mode, str) else int(mode)
+    typ = _type_of(t)
+    dims = tuple(dims) if dims is not None else (0, 0, 0)
+    cap = CONTAINER_HEADER_BYTES + max_compressed_size(t.numel(), dim, dims, typ, m)
+    out = torch.empty(cap, dtype=torch.uint8, device=t.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(t.device)
+    size = ctypes.c_size_t(0)
+    rc = lib.cuSZp_compress_container(_ptr(t), _ptr(out), cap, t.numel(), ctypes.byref(size), float(error_bound),
+                                      float(rel_bound), dim, Uint3(*dims), typ, m, _stream(stream))
+    if rc == WARNING_BOUND_EXCEEDED:
+        warnings.warn(lib.cuSZp_status_string(rc).decode(), BoundExceededWarning, stacklevel=2)
+    elif rc:
+        raise CuSZpError(rc, "compress container")
+    return out, size.value
+
+
+def decompress_container(buf, size: Optional[int] = None, stream=None):
+    """Decompress a container held in a CUDA uint8 tensor (the first `size`
+    bytes; all of it by default). Returns (values tensor, header dict); the
+    values' dtype, count and shape come from the header alone."""
+    import torch
+
+    size = buf.numel() if size is None else int(size)
+    if size < CONTAINER_HEADER_BYTES:
+        raise CuSZpError(5, "decompress container")
+    info = read_container_header(buf[:CONTAINER_HEADER_BYTES].cpu().numpy().tobytes())
+    dtype = torch.float64 if info["type"] == CUSZP_TYPE_DOUBLE else torch.float32
+    out = torch.empty(max(info["num_elements"], 1), dtype=dtype, device=buf.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(buf.device)
+    ci = ContainerInfo()
+    rc = lib.cuSZp_decompress_container(_ptr(out), out.numel(), _ptr(buf), size, ctypes.byref(ci), _stream(stream))
+    if rc:
+        raise CuSZpError(rc, "decompress container")
+    return out[:info["num_elements"]], _info_dict(ci)
+
+
+# ---------------------------------------------------------------------------
 # batched form: many arrays in one launch per direction (include/cuSZp.h,
 # cuSZp_batch_plan_*). Every array's stream is byte-identical to its
 # single-array stream.
@@ -308,7 +379,9 @@ class BatchPlan:
         sizes = (ctypes.c_size_t * max(self.count, 1))()
         st = (ctypes.c_int * max(self.count, 1))()
         rc = lib.cuSZp_batch_plan_results(self._plan, sizes, st, _stream(stream))
-        if rc and check:
+        if rc == WARNING_BOUND_EXCEEDED:
+            warnings.warn(lib.cuSZp_status_string(rc).decode(), BoundExceededWarning, stacklevel=2)
+        elif rc and check:
             raise CuSZpError(rc, "batch results")
         return list(sizes)[: self.count], list(st)[: self.count]
 

@@ -194,6 +194,11 @@ int cuSZp_scan_report(hipStream_t stream, cuszp_scan_report_t* report);
  *   stats        nonzero: count polls, waits, claims and wait time (costs one
  *                device atomic per waiting tile). */
 int cuSZp_scan_config(int steal_ticks, unsigned int hold_every, unsigned int hold_ticks, int stats);
+/* Test fault injection: the next `count` growths of a scan-state status array
+ * (single-array pool, batch plan, serving lane) fail as if the device were out
+ * of memory; the call that hits one returns CUSZP_ERROR_HIP and the next call
+ * allocates again. 0 turns it off. */
+int cuSZp_debug_fail_allocs(unsigned int count);
 
 /* ---- batched form (no reference counterpart) ----
  * Many arrays of any sizes, dtypes, modes, dims and error bounds compress (or
@@ -334,6 +339,61 @@ int cuSZp_quality(const void* d_oriData, const void* d_decData, size_t nbEle, cu
  * extent must be at least 7 (CUSZP_ERROR_INVALID_DIMS). Synchronises `stream`. */
 int cuSZp_ssim(const void* d_oriData, const void* d_decData, cuszp_dims_t dims, cuszp_type_t type, double* ssim,
                hipStream_t stream);
+
+/* ---- self-describing container (no reference counterpart) ----
+ * The reference writes header-less raw streams (examples/cuSZp.cpp:306-317 of
+ * the reference): reading one back needs the type, mode, dims, bound and
+ * length from elsewhere. A container is a 64-byte little-endian header
+ * followed by the unchanged raw stream:
+ *   [0,8)   magic 89 43 53 5A 50 0D 0A 1A ("\x89CSZP\r\n\x1a")
+ *   [8,10)  format version (1)        [10,12) header bytes (64)
+ *   [12]    dim (1..3)   [13] type (0 f32, 1 f64)   [14] mode (0..2)
+ *   [15]    flags: bit 0 = the bound was given relative to the value range
+ *   [16,28) dims x, y, z (u32 each; 0 for 1D)
+ *   [28,32) CRC-32 (IEEE 802.3, as zlib's crc32) of the 64 header bytes with
+ *           this field zero
+ *   [32,40) nbEle (u64)
+ *   [40,48) the absolute bound the stream was written with (f64; for f32
+ *           streams the float value)
+ *   [48,56) the relative bound it came from (f64; 0 when absolute)
+ *   [56,64) stream bytes (u64)
+ * The raw stream starts at byte 64 and is exactly what cuSZp_compress_ex
+ * writes, so any raw-stream reader can decompress it from there. */
+#define CUSZP_CONTAINER_HEADER_BYTES 64
+#define CUSZP_CONTAINER_VERSION 1
+typedef struct {
+    cuszp_dim_t dim;
+    cuszp_type_t type;
+    cuszp_mode_t mode;
+    int relative;                      /* 1: errorBound came from relBound */
+    cuszp_dims_t dims;                 /* {0,0,0} for 1D */
+    unsigned long long nbEle;
+    double errorBound;                 /* absolute bound of the stream */
+    double relBound;                   /* 0 unless relative */
+    unsigned long long streamBytes;    /* raw stream length after the header */
+} cuszp_container_info_t;
+/* Encodes `info` into the 64 header bytes at `header` (host memory). Rejects
+ * bad enums, dims the compressor would reject and non-positive bounds
+ * (CUSZP_ERROR_INVALID_ARGUMENT / CUSZP_ERROR_INVALID_DIMS). */
+int cuSZp_container_write_header(const cuszp_container_info_t* info, unsigned char* header);
+/* Decodes and validates a header from `len` host bytes (len >= 64): magic,
+ * version, CRC, enums, geometry and bound. Returns CUSZP_ERROR_CORRUPT_STREAM
+ * for anything that is not a valid version-1 header. */
+int cuSZp_container_read_header(const unsigned char* header, size_t len, cuszp_container_info_t* info);
+/* Compresses into a container at d_container (device memory, capacity bytes:
+ * 64 + cuSZp_max_compressed_size always suffices). relBound is recorded in
+ * the header when > 0 (errorBound must then be the absolute bound derived
+ * from it, e.g. by cuSZp_rel_error_bound). *containerSize = 64 + stream
+ * bytes. Blocking, like cuSZp_compress_ex. */
+int cuSZp_compress_container(const void* d_oriData, unsigned char* d_container, size_t capacity, size_t nbEle,
+                             size_t* containerSize, double errorBound, double relBound, cuszp_dim_t dim,
+                             cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode, hipStream_t stream);
+/* Decompresses the container of containerSize bytes at d_container (device
+ * memory) into d_decData, which holds at most decCapacity values of the
+ * container's type (CUSZP_ERROR_OUTPUT_TOO_SMALL if the container describes
+ * more). The decoded header goes to *info when info is not null. Blocking. */
+int cuSZp_decompress_container(void* d_decData, size_t decCapacity, const unsigned char* d_container,
+                               size_t containerSize, cuszp_container_info_t* info, hipStream_t stream);
 
 /* Library version string ("cuszp_amd <semver> gfx950"). */
 const char* cuSZp_version(void);

@@ -9,7 +9,8 @@ These tests make that path run on every launch:
 
   * hold: every 8th tile publishes nothing for 300 us (a bounded wait inside
     the kernel), far past the 100 us after which successors claim it;
-  * steal at once: the claim threshold is 0, so any unresolved poll claims.
+  * steal at once: the claim threshold is 0, so any unresolved poll claims
+    (with every 4th tile held for 20 us, so that polls find pending words).
 
 Every stream must still be byte-identical to the oracle and every
 decompression bit-identical, and the scan report must show that claims
@@ -25,7 +26,10 @@ from tests.workloads import grf_volume, mixed_field
 pytestmark = pytest.mark.gpu
 
 HOLD = dict(hold_every=8, hold_ticks=30_000, stats=True)   # 300 us at the 100 MHz clock
-AT_ONCE = dict(steal_ticks=0, stats=True)
+# claim at the first unresolved poll; every 4th tile also holds back for 20 us,
+# so that windows stay unresolved in both directions (a decompress tile
+# otherwise publishes its byte count within a microsecond of starting)
+AT_ONCE = dict(steal_ticks=0, hold_every=4, hold_ticks=2_000, stats=True)
 
 # (dim, dims, dtype, mode, eb): 20-50 tiles each, ragged edges in 2D/3D
 CASES = [
