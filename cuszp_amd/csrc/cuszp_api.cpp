@@ -37,6 +37,8 @@ struct Slot {
     unsigned long long* h_res = nullptr;      // pinned host copies of d_misc words
     uint32_t epoch = 0;
     unsigned info = 0;                        // information bits harvested since the last scan report
+    unsigned long long* d_offs = nullptr;     // decompression: payload offset of every decode wave
+    uint64_t cap_units = 0;
 };
 
 std::mutex g_pool_mu;
@@ -95,6 +97,26 @@ int prepare(Slot& s, uint64_t ntiles, hipStream_t st) {
         CUSZP_HIP(hipMemsetAsync(s.d_status, 0, (kStatusGuard + s.cap_tiles) * sizeof(unsigned long long), st));
         s.epoch = 1;
     }
+    return CUSZP_SUCCESS;
+}
+
+// Make room for `nunits` decompression offsets. Caller holds s.mu; a failed
+// allocation leaves no table and capacity 0.
+int prepare_offsets(Slot& s, uint64_t nunits, hipStream_t st) {
+    if (nunits <= s.cap_units) return CUSZP_SUCCESS;
+    uint64_t cap = s.cap_units * 2;
+    if (cap < nunits) cap = nunits;
+    if (cap < 32768) cap = 32768;
+    if (s.d_offs) {
+        CUSZP_HIP(hipStreamSynchronize(st));   // queued launches on this stream may still read it
+        CUSZP_HIP(hipFree(s.d_offs));
+        s.d_offs = nullptr;
+        s.cap_units = 0;
+    }
+    unsigned long long* d = nullptr;
+    CUSZP_HIP(scan_state_alloc(&d, cap * sizeof(unsigned long long)));
+    s.d_offs = d;
+    s.cap_units = cap;
     return CUSZP_SUCCESS;
 }
 
@@ -193,18 +215,24 @@ int decompress_once(void* out, const unsigned char* cmp, size_t n, size_t cmp_si
     if (cmp_size < g.H) return CUSZP_ERROR_CORRUPT_STREAM;
     const TilePlan plan = plan_tiles(dim, type, false, g);
     g.ntiles = plan.ntiles;
+    const bool f64 = type == CUSZP_TYPE_DOUBLE;
+    const uint64_t nunits = plan.ntiles * (uint64_t)plan.waves;   // decode waves
     int dev = 0;
     CUSZP_HIP(hipGetDevice(&dev));
     Slot& s = slot_for(dev, st);
     std::lock_guard<std::mutex> lk(s.mu);
-    rc = prepare(s, g.ntiles, st);
+    // the offset pass scans its segments with the slot's status words
+    rc = prepare(s, unit_segments(dim, f64, nunits), st);
+    if (!rc) rc = prepare_offsets(s, nunits, st);
     if (rc) return rc;
     const ScanCtx sc = scan_ctx(s);
     Item item = make_item(cmp, out, cmp_size, nullptr, (unsigned int*)(s.d_misc + 2), decompress_param(eb, type),
                           dim, type, mode, g);
-    hipError_t e;
-    if (type == CUSZP_TYPE_FLOAT) e = launch_decompress_f32(dim, mode, plan.waves, item, sc, st);
-    else e = launch_decompress_f64(dim, mode, plan.waves, item, sc, st);
+    hipError_t e = f64 ? launch_unit_offsets_f64(dim, mode, item, nunits, s.d_offs, sc, st)
+                       : launch_unit_offsets_f32(dim, mode, item, nunits, s.d_offs, sc, st);
+    if (e != hipSuccess) return CUSZP_ERROR_HIP;
+    e = f64 ? launch_decompress_f64(dim, mode, plan.waves, item, sc, s.d_offs, st)
+            : launch_decompress_f32(dim, mode, plan.waves, item, sc, s.d_offs, st);
     if (e != hipSuccess) return CUSZP_ERROR_HIP;
     if (!blocking) return CUSZP_SUCCESS;
     return finish_blocking(s, st, nullptr);

@@ -811,13 +811,134 @@ __device__ uint64_t header_tile_bytes(const uint8_t* __restrict__ cmp, const Geo
     return sum;
 }
 
-// ---------------------------------------------------------------------------
-// one tile per workgroup pass
-// ---------------------------------------------------------------------------
 // Lookback window of a tile: 64 * kLookbackWords predecessors per round trip.
 constexpr int kLookbackWords = 4;
 // Words in front of every status array (window reads before tile 0 land there).
 constexpr uint64_t kStatusGuard = 64 * kLookbackWords;
+
+// ---------------------------------------------------------------------------
+// Offset table of a single-array decompression: before any block is decoded,
+// a pass over the header bytes alone writes the stream offset of every
+// decode wave's payload (a "unit": the UB consecutive blocks one wave
+// decodes). The decode waves then start from their own offset, with no
+// lookback and no workgroup barrier; the header is 1/32 (1D) or 1/64 (2D/3D)
+// of the values' bytes, so the pass reads little.
+//
+// The pass is itself one single-pass decoupled-lookback scan, over segments of
+// kPrefixThreads x kPrefixBytes header bytes: thread t of segment s sums the
+// block sizes of header bytes [(s * kPrefixThreads + t) * kPrefixBytes, +kPrefixBytes),
+// which hold kPrefixBytes / UB whole units.
+// ---------------------------------------------------------------------------
+constexpr int kPrefixThreads = 256;
+constexpr int kPrefixBytes = 128;
+constexpr uint64_t kPrefixSegBytes = (uint64_t)kPrefixThreads * kPrefixBytes;
+
+// Payload bytes of the header bytes [b0, b0 + kPrefixBytes) per unit of UB
+// blocks; bytes at or past `limit` (2D/3D: the block count, so header pad
+// bytes count nothing) are blocks without payload.
+template <int MODE, int S, int UB>
+__device__ __forceinline__ void unit_sums(const uint8_t* __restrict__ cmp, uint64_t b0, uint64_t limit,
+                                          uint32_t (&sums)[kPrefixBytes / UB]) {
+    static_assert(kPrefixBytes % UB == 0 && UB % 16 == 0, "whole units of 16-byte words");
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    typedef uint32_t v4uu __attribute__((ext_vector_type(4), aligned(1)));
+    constexpr int V = kPrefixBytes / 16;
+    v4u w[V];
+    if (b0 + kPrefixBytes <= limit) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) w[i] = __builtin_nontemporal_load((const v4uu*)(cmp + b0) + i);
+    } else {
+        uint32_t* wb = (uint32_t*)w;
+#pragma unroll
+        for (int i = 0; i < 4 * V; ++i) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t b = b0 + 4 * i + k;
+                v |= (b < limit ? (uint32_t)cmp[b] : 0u) << (8 * k);
+            }
+            wb[i] = v;
+        }
+    }
+    const uint32_t* wb = (const uint32_t*)w;
+#pragma unroll
+    for (int u = 0; u < kPrefixBytes / UB; ++u) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int i = 0; i < UB / 4; ++i) {
+            const uint32_t x = wb[u * (UB / 4) + i];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s += view_of<MODE, S>((x >> (8 * k)) & 0xffu).size;
+        }
+        sums[u] = s;
+    }
+}
+
+// Payload bytes of segment `seg`, by one wave (a waiting wave's recomputation
+// of a predecessor it claimed; see wave_lookback).
+template <int MODE, int S>
+__device__ uint64_t segment_bytes(const uint8_t* __restrict__ cmp, uint64_t limit, uint64_t seg, int lane) {
+    uint64_t sum = 0;
+    for (uint32_t i = (uint32_t)lane; i < kPrefixSegBytes; i += kWaveSize) {
+        const uint64_t b = seg * kPrefixSegBytes + i;
+        sum += view_of<MODE, S>(b < limit ? (uint32_t)cmp[b] : 0u).size;
+    }
+    return wave_sum_u40(sum);
+}
+
+// offs[u] = H + payload bytes of units 0 .. u-1, for u < nunits; segment =
+// base + blockIdx.x of nseg. With `total`, the last segment writes the stream
+// length the header describes, H + all payload bytes.
+template <int MODE, int S, int UB>
+__global__ void __launch_bounds__(kPrefixThreads)
+unit_offsets_kernel(const uint8_t* __restrict__ cmp, const uint64_t limit, const uint64_t H, const uint64_t nunits,
+                    unsigned long long* __restrict__ offs, unsigned long long* __restrict__ total,
+                    const ScanCtx sc, const uint64_t nseg, const uint64_t base) {
+    constexpr int UPT = kPrefixBytes / UB;   // units per thread
+    constexpr int NW = kPrefixThreads / kWaveSize;
+    __shared__ uint32_t wsum[NW];
+    __shared__ uint64_t excl;
+    const uint64_t seg = base + blockIdx.x;
+    const int t = (int)threadIdx.x, lane = (int)__lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint64_t th = seg * kPrefixThreads + (uint64_t)t;
+    uint32_t sums[UPT];
+    unit_sums<MODE, S, UB>(cmp, th * kPrefixBytes, limit, sums);
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) tot += sums[k];
+    const uint32_t incl = scan_wave(tot);   // < 2^32: a segment holds at most 2^15 blocks of <= 2 KiB
+    if (lane == kWaveSize - 1) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t wex = 0, stot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const uint32_t v = wsum[w];
+        wex += (w < wave) ? v : 0u;
+        stot += v;
+    }
+    if (wave == 0) {
+        const uint64_t e = wave_lookback<kLookbackWords>(sc, seg, 0, stot, lane, [&](uint64_t p) {
+            return segment_bytes<MODE, S>(cmp, limit, p, lane);
+        });
+        if (lane == 0) {
+            excl = e;
+            if (seg == nseg - 1 && total) *total = H + e + stot;
+        }
+    }
+    __syncthreads();
+    uint64_t o = H + excl + wex + (incl - tot);
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+        const uint64_t u = th * UPT + (uint64_t)k;
+        if (u < nunits) offs[u] = o;
+        o += sums[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one tile per workgroup pass
+// ---------------------------------------------------------------------------
 
 // Tile shape. A wave works through kChunks chunks of ShapeC<T>::G groups (one
 // element per lane per group); a tile is WAVES * kChunks * G groups (16384 f32
@@ -1314,9 +1435,15 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
 //      stream words (the next chunk's loads are issued while this chunk
 //      decodes), transposes them back, rebuilds q along the Lorenzo chains and
 //      stores (T)q * eb * 2.
-template <int DIM, int MODE, class T, int WAVES, int LB = kLookbackWords>
+//
+// OFFS (single-array launches): the wave's payload offset comes from the
+// offset table `offs` written by unit_offsets_kernel, so step 1 is only the
+// wave's own header bytes: no lookback and no workgroup barrier, every wave
+// runs on its own.
+template <int DIM, int MODE, class T, int WAVES, int LB = kLookbackWords, bool OFFS = false>
 __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t tile, const ScanCtx& sc,
-                                                SmemD<WAVES>& sm) {
+                                                SmemD<WAVES>& sm,
+                                                const unsigned long long* __restrict__ offs = nullptr) {
     constexpr int G = ShapeC<T>::G;
     constexpr int S = (DIM == 1) ? 4 : 8;
     constexpr int BPG = (DIM == 1) ? 2 : 1;
@@ -1346,26 +1473,35 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         wsum += ok ? view_of<MODE, S>(byte).size : 0u;
     }
     const uint32_t wtot = rdlane(scan_wave(wsum), kWaveSize - 1);   // < 2^32; a cross-lane scan: all lanes
-    if (lane == 0) sm.wtot[wave] = wtot;
-    __syncthreads();
-    uint64_t wexcl = 0, ttot = 0;
+    uint64_t wave_base;
+    if constexpr (OFFS) {
+        (void)sc;
+        wave_base = uniform64(offs[lt * WAVES + (uint64_t)wave]);
+        __builtin_amdgcn_wave_barrier();   // the header bytes in LDS are this wave's own
+    } else {
+        if (lane == 0) sm.wtot[wave] = wtot;
+        __syncthreads();
+        uint64_t wexcl = 0, ttot = 0;
 #pragma unroll
-    for (int w = 0; w < WAVES; ++w) {
-        const uint64_t t = sm.wtot[w];
-        wexcl += (w < wave) ? t : 0;
-        ttot += t;
+        for (int w = 0; w < WAVES; ++w) {
+            const uint64_t t = sm.wtot[w];
+            wexcl += (w < wave) ? t : 0;
+            ttot += t;
+        }
+        // wave-uniform: kept in scalar registers across the lookback
+        wexcl = uniform64(wexcl);
+        ttot = uniform64(ttot);
+        if (wave == 0) {
+            const uint64_t e = wave_lookback<LB>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
+                return header_tile_bytes<DIM, MODE, kChunks * G, WAVES>(cmp, geo, p - cur.tile0, lane);
+            });
+            if (lane == 0) sm.excl = e;
+        }
+        __syncthreads();
+        wave_base = geo.H + sm.excl + wexcl;
     }
-    if (wave == 0) {
-        const uint64_t e = wave_lookback<LB>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
-            return header_tile_bytes<DIM, MODE, kChunks * G, WAVES>(cmp, geo, p - cur.tile0, lane);
-        });
-        if (lane == 0) sm.excl = e;
-    }
-    __syncthreads();
-
-    const uint64_t wave_base = geo.H + sm.excl + wexcl;
     const uint8_t* const wbase = cmp + uniform64(wave_base);
-    const bool intact = wave_base + sm.wtot[wave] <= cur.limit;
+    const bool intact = wave_base + wtot <= cur.limit;
     if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
     // The wave's stream words go through a buffer resource over its payload:
     // 32-bit offsets, and an offset past the range (kAbsent) reads 0, so lanes
@@ -1617,11 +1753,14 @@ compress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
     compress_tile<DIM, MODE, T, WAVES, 2>(item, base + blockIdx.x, sc, sm);
 }
 
+// Waves take their payload offsets from `offs` (unit_offsets_kernel, launched
+// before on the same stream); sc is unused.
 template <int DIM, int MODE, class T, int WAVES>
 __global__ void __launch_bounds__(WAVES * kWaveSize, kDecompressWavesPerSimd)
-decompress_kernel(const Item item, const ScanCtx sc, const uint64_t base) {
+decompress_kernel(const Item item, const ScanCtx sc, const unsigned long long* __restrict__ offs,
+                  const uint64_t base) {
     __shared__ SmemD<WAVES> sm;
-    decompress_tile<DIM, MODE, T, WAVES>(item, base + blockIdx.x, sc, sm);
+    decompress_tile<DIM, MODE, T, WAVES, kLookbackWords, true>(item, base + blockIdx.x, sc, sm, offs);
 }
 
 // ---------------------------------------------------------------------------

@@ -21,8 +21,26 @@ TilePlan plan_tiles(int dim, int type, bool compress, const Geom& g);
 // Single-array launches; item.param is 0.5/eb (compress) or eb (decompress).
 hipError_t launch_compress_f32(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st);
 hipError_t launch_compress_f64(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st);
-hipError_t launch_decompress_f32(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st);
-hipError_t launch_decompress_f64(int dim, int mode, int waves, const Item& item, const ScanCtx& sc, hipStream_t st);
+// Single-array decompression reads each wave's payload offset from `offs`,
+// which launch_unit_offsets_* fills first on the same stream.
+hipError_t launch_decompress_f32(int dim, int mode, int waves, const Item& item, const ScanCtx& sc,
+                                 const unsigned long long* offs, hipStream_t st);
+hipError_t launch_decompress_f64(int dim, int mode, int waves, const Item& item, const ScanCtx& sc,
+                                 const unsigned long long* offs, hipStream_t st);
+// Offset table of a single-array decompression (unit_offsets_kernel): offs[u]
+// for the nunits decode waves of `item`'s plan, from the header bytes of the
+// stream at item.src; sc's status words serve the pass's own scan (one per
+// segment of kPrefixSegBytes header bytes).
+hipError_t launch_unit_offsets_f32(int dim, int mode, const Item& item, uint64_t nunits, unsigned long long* offs,
+                                   const ScanCtx& sc, hipStream_t st);
+hipError_t launch_unit_offsets_f64(int dim, int mode, const Item& item, uint64_t nunits, unsigned long long* offs,
+                                   const ScanCtx& sc, hipStream_t st);
+// Blocks one decode wave handles (kChunks chunks of G groups).
+inline uint64_t unit_blocks(int dim, bool f64) { return (uint64_t)kChunks * (f64 ? 4 : 8) * (dim == 1 ? 2 : 1); }
+// Segments of the offset pass over `nunits` units.
+inline uint64_t unit_segments(int dim, bool f64, uint64_t nunits) {
+    return (nunits * unit_blocks(dim, f64) + kPrefixSegBytes - 1) / kPrefixSegBytes;
+}
 
 // Batched launches over a device table of items (ordered by tile0), one
 // workgroup per tile; the scan tag is read from *epoch when the kernel starts

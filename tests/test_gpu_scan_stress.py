@@ -80,9 +80,52 @@ def test_single_array_claims(device, oracle, stress, case, knobs):
     rc = P.scan_report()
     assert got == want
     assert rc["stolen"] and rc["claims"] > 0, rc
+    # these streams are a single segment of the decompression's offset pass
+    # (32 KiB of header bytes): nothing to claim, the result must not change
+    y = gpu_decompress(want, n, dtype, eb, mode, dim, dims)
+    yref = oracle.decompress(want, n, dtype, eb, mode, dim, dims)
+    assert np.array_equal(y.view(np.uint8), yref.view(np.uint8))
+
+
+# Streams of 10-12 segments of the decompression's offset pass (a segment is
+# 32,768 header bytes, i.e. 32,768 blocks), so that held segments have
+# successors that claim them: (dim, dims, dtype, mode, eb)
+OFFSET_CASES = [
+    (1, None, np.float32, 1, 1e-3),                # 10,485,767 values: 10.0 segments
+    (1, None, np.float64, 2, 1e-4),
+    (2, (2048, 1160, 8), np.float32, 2, 1e-2),     # 256 x 145 x 8 = 296,960 blocks
+    (3, (256, 256, 300), np.float64, 0, 1e-3),     # 64 x 64 x 75 = 307,200 blocks
+]
+
+
+def _offset_input(case):
+    dim, dims, dtype, mode, eb = case
+    if dim == 1:
+        return mixed_field(10 * 1024 * 1024 + 7, seed=9, dtype=dtype), (0, 0, 0)
+    x, y, z = dims
+    i = np.arange(x * y * z, dtype=np.float64)
+    rng = np.random.default_rng(10)
+    v = np.sin(i * 3e-5) * 40 + np.sin(i * 7e-3) * 3 + rng.normal(0, 0.05, i.size)
+    return v.astype(dtype), dims
+
+
+@pytest.mark.parametrize("knobs", [HOLD, AT_ONCE], ids=["hold", "steal-at-once"])
+@pytest.mark.parametrize("case", OFFSET_CASES, ids=[_ids(c) for c in OFFSET_CASES])
+def test_offset_pass_claims(device, oracle, stress, case, knobs):
+    """Single-array decompression: its offset pass (the scan of the header's
+    block sizes) under the same stress; the decompressions equal the oracle's
+    and claims are reported."""
+    from cuszp_amd import pycuSZp as P
+
+    dim, _, dtype, mode, eb = case
+    x, dims = _offset_input(case)
+    n = x.size
+    want = oracle.compress(x, eb, mode, dim, dims, nthreads=oracle.max_threads())
+    print(f"offset pass stress {_ids(case)} {knobs} n={n}", flush=True)
+    stress(**knobs)
     y = gpu_decompress(want, n, dtype, eb, mode, dim, dims)
     rd = P.scan_report()
-    yref = oracle.decompress(want, n, dtype, eb, mode, dim, dims)
+    yref = oracle.decompress(want, n, dtype, eb, mode, dim, dims, nthreads=oracle.max_threads())
     assert np.array_equal(y.view(np.uint8), yref.view(np.uint8))
     assert rd["stolen"] and rd["claims"] > 0, rd
 
