@@ -64,6 +64,7 @@ EXPORTED = (
         "cuSZp_container_read_header",
         "cuSZp_compress_container",
         "cuSZp_decompress_container",
+        "cuSZp_compress_noalloc",
     ]
 )
 
@@ -233,6 +234,8 @@ def load() -> ctypes.CDLL:
     L.cuSZp_compress_container.restype = i32
     L.cuSZp_decompress_container.argtypes = [vp, sz, vp, sz, pci, vp]
     L.cuSZp_decompress_container.restype = i32
+    L.cuSZp_compress_noalloc.argtypes = [vp, vp, sz, sz, vp, f64, i32, Uint3, i32, i32, vp]
+    L.cuSZp_compress_noalloc.restype = i32
     L.cuSZp_value_range.argtypes = [vp, sz, i32, ctypes.POINTER(f64), ctypes.POINTER(f64), vp]
     L.cuSZp_value_range.restype = i32
     L.cuSZp_rel_error_bound.argtypes = [vp, sz, i32, f32, ctypes.POINTER(f32), vp]

@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "libcuszp_amd.so")
 CLI = os.path.join(HERE, "cuszp")
 SOURCES = ["kern_batch_c.hip", "kern_batch_d.hip", "kern_f32.hip", "kern_f64.hip", "kern_range.hip", "kern_quality.hip",
            "cuszp_api.cpp",
-           "cuszp_batch.cpp", "cuszp_serve.cpp", "cuszp_container.cpp"]
+           "cuszp_batch.cpp", "cuszp_serve.cpp", "cuszp_container.cpp",
+           "cuszp_noalloc.hip"]
 ARCH = os.environ.get("CUSZP_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

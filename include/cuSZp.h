@@ -395,6 +395,25 @@ int cuSZp_compress_container(const void* d_oriData, unsigned char* d_container, 
 int cuSZp_decompress_container(void* d_decData, size_t decCapacity, const unsigned char* d_container,
                                size_t containerSize, cuszp_container_info_t* info, hipStream_t stream);
 
+/* ---- compression that allocates nothing (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops and harnesses that forbid hipMalloc. Same stream as
+ * cuSZp_compress_ex, but the scan state lives in a fixed device array of the
+ * library (about 514 KiB per GPU, held until the process ends) and the call
+ * only launches kernels: no hipMalloc/hipFree, no memset, no copy, no
+ * synchronisation. When the launches have finished, d_result[0] holds the
+ * stream length and d_result[1] the call's status (CUSZP_SUCCESS,
+ * CUSZP_ERROR_OUTPUT_TOO_SMALL, CUSZP_ERROR_INTERNAL or
+ * CUSZP_WARNING_BOUND_EXCEEDED); d_result is two device words of the caller.
+ * At most 2^16 tiles: 2^31 f32 or 2^30 f64 values; larger arrays return
+ * CUSZP_ERROR_TOO_LARGE before anything is launched. Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * host threads) race, so the caller must serialise them. */
+int cuSZp_compress_noalloc(const void* d_oriData, unsigned char* d_cmpBytes, size_t cmpCapacity,
+                           size_t nbEle, unsigned long long* d_result, double errorBound,
+                           cuszp_dim_t dim, cuszp_dims_t dims, cuszp_type_t type, cuszp_mode_t mode,
+                           hipStream_t stream);
+
 /* Library version string ("cuszp_amd <semver> gfx950"). */
 const char* cuSZp_version(void);
 
