@@ -14,13 +14,13 @@
 //     The grid has one workgroup per tile (tile = launch base + blockIdx.x).
 //     No ticket atomics: one hot device-scope atomic serialises at about
 //     12 ns per operation on MI355X;
-//   * lanes hold elements in "mirror" order: lane L holds element L^7, which is
-//     exactly the bit order of cuSZp's sign/plane bytes (element e at byte e/8,
-//     bit 7-e%8). Each lane packs its sign into bit 31 above the 31-bit
-//     magnitude; a 32x32 bit-matrix transpose across each 32-lane half (five
-//     ds_swizzle xor stages) then leaves plane p's stream word in lane p and the
-//     sign word in lane 31 -- for every rate at a fixed cost. Decompression runs
-//     the same transpose (it is an involution);
+//   * lane e holds element e of the group. Each lane packs its sign into bit 31
+//     above the 31-bit magnitude; a 32x32 bit-matrix transpose across each
+//     32-lane half (five ds_swizzle xor stages) then leaves plane p's stream
+//     word in lane p and the sign word in lane 31 -- for every rate at a fixed
+//     cost. The transpose also puts the elements in the bit order of cuSZp's
+//     sign/plane bytes (element e at byte e/8, bit 7-e%8, i.e. bit e^7 of the
+//     word) at no extra cost; decompression runs the inverse transpose;
 //   * per-block bookkeeping (rates, header bytes, sizes, offsets) runs once per
 //     wave with one lane per block;
 //   * the inter-tile exclusive scan of compressed bytes is a single-pass
@@ -155,18 +155,27 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// 32x32 bit transpose within each 32-lane half: out lane j bit i = in lane i bit j.
-// Stage J exchanges lane-index bit log2(J) with data bit log2(J): the lane whose
-// bit is 0 keeps its data bits with that bit 0 and takes its partner's (lane^J)
-// other half shifted into place. Stages 16 and 8 move whole bytes (v_perm),
-// stages 4, 2, 1 rotate and merge (v_alignbit + v_bfi). The partner value comes
-// from ds_swizzle (xor mode), which runs on the LDS pipe, not the VALU.
+// 32x32 bit transposes within each 32-lane half, with the element order of
+// cuSZp's sign/plane bytes folded in. In the stream, element e of a 32-element
+// half sits at bit e^7 of its plane word (byte e/8, bit 7-e%8). Lanes hold
+// elements in natural order, so:
+//   compress   (ENC): out lane i bit j = in lane j^7 bit i   (lane e's word -> plane words)
+//   decompress (DEC): out lane i bit j = in lane j bit i^7   (plane words -> lane e's word)
+// Stage J exchanges lane-index bit log2(J) with data bit log2(J). Stages 16 and
+// 8 move whole bytes (v_perm); stages 4, 2, 1 rotate the lane's own word and
+// merge in its partner's (v_alignbit + v_bfi). Exchanging the bit with its
+// complement in those three stages (the order twist e -> e^7) costs nothing:
+// it only changes which half of each pair stays in place, i.e. the per-lane
+// shift and mask constants. The partner value (lane^J) comes from ds_swizzle
+// (xor mode), which runs on the LDS pipe, not the VALU.
 // ---------------------------------------------------------------------------
 struct XposeConsts {
     uint32_t sh1, mk1, sh2, mk2, sh4, mk4, sel8, sel16;
-    __device__ __forceinline__ void init(int lane) {
+    // dec: the decompress transpose (plane words -> elements)
+    __device__ __forceinline__ void init(int lane, bool dec) {
         const int l = lane & 31;
-        const bool lo1 = !(l & 1), lo2 = !(l & 2), lo4 = !(l & 4), lo8 = !(l & 8), lo16 = !(l & 16);
+        const bool lo1 = !(l & 1) != dec, lo2 = !(l & 2) != dec, lo4 = !(l & 4) != dec;
+        const bool lo8 = !(l & 8), lo16 = !(l & 16);
         sh1 = lo1 ? 31 : 1;  mk1 = lo1 ? 0xAAAAAAAAu : 0x55555555u;
         sh2 = lo2 ? 30 : 2;  mk2 = lo2 ? 0xCCCCCCCCu : 0x33333333u;
         sh4 = lo4 ? 28 : 4;  mk4 = lo4 ? 0xF0F0F0F0u : 0x0F0F0F0Fu;
@@ -188,14 +197,15 @@ __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b) {
 __device__ __forceinline__ uint32_t xpose32(uint32_t x, const XposeConsts& c) {
     x = __builtin_amdgcn_perm(swizzle_xor<16>(x), x, c.sel16);
     x = __builtin_amdgcn_perm(swizzle_xor<8>(x), x, c.sel8);
-    x = bfi(c.mk4, rotr(swizzle_xor<4>(x), c.sh4), x);
-    x = bfi(c.mk2, rotr(swizzle_xor<2>(x), c.sh2), x);
-    x = bfi(c.mk1, rotr(swizzle_xor<1>(x), c.sh1), x);
+    x = bfi(c.mk4, rotr(x, c.sh4), swizzle_xor<4>(x));
+    x = bfi(c.mk2, rotr(x, c.sh2), swizzle_xor<2>(x));
+    x = bfi(c.mk1, rotr(x, c.sh1), swizzle_xor<1>(x));
     return x;
 }
 // The same transpose on N independent values, stage by stage, so the N
 // swizzles of a stage are in flight together (one LDS round trip per stage
-// instead of one per value).
+// instead of one per value); the lanes' own rotations are computed while the
+// swizzles are in flight.
 template <int N>
 __device__ __forceinline__ void xpose32_n(uint32_t* x, const XposeConsts& c) {
     uint32_t t[N];
@@ -210,15 +220,22 @@ __device__ __forceinline__ void xpose32_n(uint32_t* x, const XposeConsts& c) {
 #pragma unroll
     for (int i = 0; i < N; ++i) t[i] = swizzle_xor<4>(x[i]);
 #pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk4, rotr(t[i], c.sh4), x[i]);
+    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk4, rotr(x[i], c.sh4), t[i]);
 #pragma unroll
     for (int i = 0; i < N; ++i) t[i] = swizzle_xor<2>(x[i]);
 #pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk2, rotr(t[i], c.sh2), x[i]);
+    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk2, rotr(x[i], c.sh2), t[i]);
 #pragma unroll
     for (int i = 0; i < N; ++i) t[i] = swizzle_xor<1>(x[i]);
 #pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk1, rotr(t[i], c.sh1), x[i]);
+    for (int i = 0; i < N; ++i) x[i] = bfi(c.mk1, rotr(x[i], c.sh1), t[i]);
+}
+// Bit j of each 32-bit half of m moved to bit j^7 (each byte bit-reversed):
+// a per-element mask in natural order as a sign word in stream order.
+__device__ __forceinline__ uint64_t stream_order64(uint64_t m) {
+    const uint32_t lo = __builtin_bswap32(__builtin_bitreverse32((uint32_t)m));
+    const uint32_t hi = __builtin_bswap32(__builtin_bitreverse32((uint32_t)(m >> 32)));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // Inclusive prefix sums (wrapping u32).
@@ -973,6 +990,10 @@ constexpr int kXposeBatch = 4;
 // conflicts), so the global stores write whole rows 16 bytes per lane.
 constexpr int kOutPad = 16;
 constexpr int kOutStage = 16 * (128 + kOutPad);   // the larger of 16 x 128 B and 8 x 256 B, padded
+// Largest chunk payload (from its 4-byte-aligned start) read with dense loads
+// into the LDS buffer of a decompress wave; kOutStage bytes hold it.
+constexpr int kDenseBytes = 2048;
+static_assert(kDenseBytes <= kOutStage, "dense chunk buffer");
 
 // Shared memory of a compress workgroup.
 template <int WAVES>
@@ -982,17 +1003,51 @@ struct SmemC {
     uint32_t over[WAVES];                      // the wave's payload did not fit its staging area
     Meta meta[WAVES][32];                      // per-block metadata of the chunk in work
     uint32_t aux[WAVES][32];                   // sign words of the chunk's 32-plane 2D/3D groups
+    uint8_t hdr[WAVES][128];                   // header bytes of the wave's blocks, stored after its chunks
     uint4 stage[WAVES][kStageBytes / 16 + 1];  // the wave's payload, as it will appear in the stream (+ a spare word)
 };
 
 // Shared memory of a decompress workgroup.
+// Per-block metadata of a decompress wave, in LDS (one 8-byte entry per block
+// of the wave, written once by the wave's bookkeeping):
+//   a:  [19:0] byte offset of the block's sign word within the wave's payload
+//       (past any outlier prefix), [30] 32 planes (2D/3D only: the sign words
+//       move in front of the planes), [31] outlier-selected
+//   nb: bit l CLEAR when lane l of a 32-lane half carries one of the block's
+//       stream words (planes 0..R-1 in lanes 0..R-1, the sign word in lane 31)
+struct DMeta {
+    uint32_t a, nb;
+    __device__ __forceinline__ uint32_t off() const { return a & 0xfffffu; }
+    __device__ __forceinline__ bool wide() const { return (a >> 30) & 1; }
+    __device__ __forceinline__ bool sel() const { return (int32_t)a < 0; }
+    __device__ __forceinline__ bool nz() const { return nb != ~0u; }
+    // Buffer offset of lane l's word, wo bytes into the block (a block without
+    // 32 planes): bits 30..0 from the offset, bit 31 set for a lane without a
+    // word, i.e. past every buffer range (reads 0). Three VALU operations.
+    __device__ __forceinline__ uint32_t word_at(uint32_t l, uint32_t wo) const {
+        return bfi(0x7fffffffu, a + wo, nb << (31u - l));
+    }
+};
+template <int DIM>
+__device__ __forceinline__ DMeta make_dmeta(uint32_t off, const BlockView& v, bool nz) {
+    DMeta m;
+    const bool wide = DIM != 1 && v.R >= 32;
+    m.a = (off + v.head) | ((uint32_t)wide << 30) | (v.sel << 31);
+    m.nb = ~(!nz ? 0u : v.R >= 32 ? ~0u : (((1u << v.R) - 1u) | 0x80000000u));
+    return m;
+}
+
+// Blocks of metadata per decompress wave (its kChunks * blocks-per-chunk).
+constexpr int kMetaPerWave = 128;
 template <int WAVES>
 struct SmemD {
     uint64_t wtot[WAVES];             // per-wave byte counts
     uint64_t excl;                    // the tile's exclusive prefix (wave 0's lookback)
-    Meta meta[2][WAVES][32];          // per-block metadata of the chunk in work and the next one
-    uint8_t hdr[WAVES][128];          // header bytes of the wave's blocks
-    uint4 ostage[WAVES][kOutStage / 16];   // 2D/3D: a chunk's decoded rows, padded
+    DMeta meta[WAVES][kMetaPerWave];  // per-block metadata of the wave's blocks
+    // a chunk's stream words, loaded densely (then read per lane); 2D/3D: after
+    // that the chunk's decoded rows, padded
+    uint4 ostage[WAVES][kOutStage / 16];
+    uint32_t zero[WAVES][2];          // two zero words per wave (read by lanes without a word)
 };
 
 // Load of one element per lane. 1D loads cover whole cache lines and stream
@@ -1097,7 +1152,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
     const int pl = pred_lane<DIM>(lane);
     const int pl_src = pl < 0 ? lane : pl;
     XposeConsts xc;
-    xc.init(lane);
+    xc.init(lane, false);
     // after the transpose, element 0 of a block is bit 7 of its plane words: in
     // both halves for 1D (two blocks), only in the low half for 2D/3D
     const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
@@ -1153,18 +1208,17 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                             d = q - qp;
                         }
                         d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
-                        const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
-                        const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
+                        const uint32_t a = ((int32_t)d < 0) ? 0u - d : d;
                         if constexpr (DIM != 1) anywide |= ballot((int32_t)a < 0);
                         // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
                         // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
-                        W[g] = bfi(0x80000000u, dm, a);
-                        if constexpr (MODE == kOutlier) {
+                        W[g] = bfi(0x80000000u, d, a);
+                        if constexpr (MODE == kOutlier) {   // |d| of each block's element 0
                             if constexpr (DIM == 1) {
-                                bko = wrlane<2 * g>(rdlane(a, 7), bko);
-                                bko = wrlane<2 * g + 1>(rdlane(a, 39), bko);
+                                bko = wrlane<2 * g>(rdlane(a, 0), bko);
+                                bko = wrlane<2 * g + 1>(rdlane(a, 32), bko);
                             } else {
-                                bko = wrlane<g>(rdlane(a, 7), bko);
+                                bko = wrlane<g>(rdlane(a, 0), bko);
                             }
                         }
                     });
@@ -1207,8 +1261,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                         d = q - qp;
                     }
                     d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
-                    const uint32_t dm = dpp<kDppHalfMirror>(d);   // lane L now holds element L^7
-                    const uint32_t a = ((int32_t)dm < 0) ? 0u - dm : dm;
+                    const uint32_t a = ((int32_t)d < 0) ? 0u - d : d;
                     uint64_t m1, m2 = 0;
                     bool wide = false;
                     if constexpr (DIM != 1) {
@@ -1218,14 +1271,14 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                     if (!wide) {
                         // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
                         // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
-                        W[g] = xpose32(bfi(0x80000000u, dm, a), xc);
+                        W[g] = xpose32(bfi(0x80000000u, d, a), xc);
                         m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
                         if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
                     } else {
                         W[g] = xpose32(a, xc);
                         m1 = ballot(W[g] != 0);
                         if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0);
-                        const uint64_t sg = ballot((int32_t)dm < 0);
+                        const uint64_t sg = stream_order64(ballot((int32_t)d < 0));   // the sign words
                         if (lane == 0) {
                             sm.aux[wave][2 * g] = (uint32_t)sg;
                             sm.aux[wave][2 * g + 1] = (uint32_t)(sg >> 32);
@@ -1238,14 +1291,14 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                         if constexpr (MODE == kOutlier) {
                             bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
                             bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
-                            bko = wrlane<2 * g>(rdlane(a, 7), bko);
-                            bko = wrlane<2 * g + 1>(rdlane(a, 39), bko);
+                            bko = wrlane<2 * g>(rdlane(a, 0), bko);
+                            bko = wrlane<2 * g + 1>(rdlane(a, 32), bko);
                         }
                     } else {
                         bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
                         if constexpr (MODE == kOutlier) {
                             bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
-                            bko = wrlane<g>(rdlane(a, 7), bko);
+                            bko = wrlane<g>(rdlane(a, 0), bko);
                         }
                     }
                 });
@@ -1280,11 +1333,10 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             const uint32_t incl = scan_wave(sz);
             const uint32_t ctot = rdlane(incl, kWaveSize - 1);
             const uint32_t off = wave_tot + incl - sz;   // block offset within the wave's payload
-            if (!direct) {
-                // header bytes (lane j: block j); 2D/3D pad bytes [nblocks, H) are 0
-                const uint64_t b0 = (DIM == 1) ? 2 * grp0 : grp0;
-                if (blk && (DIM == 1 || b0 + (uint64_t)lane < geo.H)) dst[b0 + (uint64_t)lane] = (uint8_t)byte;
-            }
+            // header bytes (lane j: block j) go to the stream after the wave's
+            // chunks: no global store inside the chunk loop, whose waits for
+            // the rolling loads would otherwise also wait for those stores
+            if (!direct && blk) sm.hdr[wave][c * NB + lane] = (uint8_t)byte;
             const bool put = direct || (!over && wave_tot + ctot <= kStageBytes);
             over |= !put;
             if (put) {
@@ -1371,8 +1423,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             for (int c = 0; c < kChunks; ++c) {
                 const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
                 if (grp0 >= gvalid) {
-                    const uint64_t b0 = (DIM == 1) ? 2 * grp0 : grp0;
-                    if (!direct && lane < NB && (DIM == 1 || b0 + (uint64_t)lane < geo.H)) dst[b0 + (uint64_t)lane] = 0;
+                    if (!direct && lane < NB) sm.hdr[wave][c * NB + lane] = 0;
                     continue;
                 }
                 T xv[G];
@@ -1382,6 +1433,22 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
             }
         }
         if (direct) return;
+
+        // ---- the wave's header bytes (their positions do not depend on the
+        // offset); 2D/3D pad bytes [nblocks, H) are 0 -------------------------------
+        {
+            constexpr int NBW = kChunks * NB;
+            constexpr int BPL = (NBW + kWaveSize - 1) / kWaveSize;
+            static_assert(NBW <= 128, "header bytes of a wave");
+            __builtin_amdgcn_wave_barrier();
+            const uint64_t wb0 = (DIM == 1) ? 2 * wgrp0 : wgrp0;
+#pragma unroll
+            for (int k = 0; k < BPL; ++k) {
+                const int j = BPL * lane + k;
+                const uint64_t b = wb0 + (uint64_t)j;
+                if (j < NBW && (DIM == 1 || b < geo.H)) dst[b] = sm.hdr[wave][j];
+            }
+        }
 
         // ---- the tile's offset (wave 0) ---------------------------------------------
         if (lane == 0) {
@@ -1428,18 +1495,18 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
 // Decompresses tile `tile` of array `cur` (every wave of the workgroup calls
 // it); the tile has the compress tile's shape (kChunks chunks of G groups per
 // wave):
-//   1. each wave reads the header bytes of its blocks (parked in LDS) and sums
-//      their sizes; the wave totals combine in LDS and wave 0 runs the lookback;
-//   2. each wave works through its chunks: bookkeeping lanes (one per block)
-//      turn header bytes into offsets and metadata, every lane loads its
-//      stream words (the next chunk's loads are issued while this chunk
-//      decodes), transposes them back, rebuilds q along the Lorenzo chains and
-//      stores (T)q * eb * 2.
+//   1. each wave reads the header bytes of its blocks (one or two per lane),
+//      scans their sizes once and writes every block's metadata (offset within
+//      the wave's payload, lane mask) to LDS;
+//   2. the wave's payload offset: from the offset table (OFFS), else the wave
+//      totals combine in LDS and wave 0 runs the lookback;
+//   3. each wave works through its chunks: every lane loads its stream words
+//      (the next chunk's loads are issued while this chunk decodes), transposes
+//      them back, rebuilds q along the Lorenzo chains and stores (T)q * eb * 2.
 //
 // OFFS (single-array launches): the wave's payload offset comes from the
-// offset table `offs` written by unit_offsets_kernel, so step 1 is only the
-// wave's own header bytes: no lookback and no workgroup barrier, every wave
-// runs on its own.
+// offset table `offs` written by unit_offsets_kernel: no lookback and no
+// workgroup barrier, every wave runs on its own.
 template <int DIM, int MODE, class T, int WAVES, int LB = kLookbackWords, bool OFFS = false>
 __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t tile, const ScanCtx& sc,
                                                 SmemD<WAVES>& sm,
@@ -1447,10 +1514,10 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     constexpr int G = ShapeC<T>::G;
     constexpr int S = (DIM == 1) ? 4 : 8;
     constexpr int BPG = (DIM == 1) ? 2 : 1;
-    constexpr int NB = BPG * G;              // blocks per chunk
-    constexpr int NBW = kChunks * NB;        // blocks per wave
-    static_assert(NBW <= 2 * kWaveSize, "header bytes of a wave: at most two per lane");
-    static_assert(NB <= 32, "shared metadata sized for 32 blocks per chunk");
+    constexpr int NB = BPG * G;                         // blocks per chunk
+    constexpr int NBW = kChunks * NB;                   // blocks per wave: 128, 64 or 32
+    constexpr int BPL = (NBW + kWaveSize - 1) / kWaveSize;   // blocks per bookkeeping lane
+    static_assert(NBW <= 2 * kWaveSize && NBW <= kMetaPerWave, "a wave's blocks: at most two per lane");
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1461,23 +1528,44 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     const uint64_t wgrp0 = (lt * WAVES + (uint64_t)wave) * (kChunks * G);   // this wave's first group
     const uint64_t wb0 = (DIM == 1) ? 2 * wgrp0 : wgrp0;                     // and first block
 
-    // ---- header bytes of the wave's blocks -------------------------------------
-    uint32_t wsum = 0;
+    // ---- bookkeeping: lane j holds blocks BPL*j .. BPL*j + BPL-1 of the wave ------
+    BlockView v[BPL];
+    uint32_t sz[BPL];
+    bool okb[BPL];
+    uint32_t lsum = 0, r32 = 0;
 #pragma unroll
-    for (int i = 0; i < (NBW + kWaveSize - 1) / kWaveSize; ++i) {
-        const int j = i * kWaveSize + lane;
+    for (int k = 0; k < BPL; ++k) {
+        const int j = BPL * lane + k;
         const uint64_t b = wb0 + (uint64_t)j;
-        const bool ok = j < NBW && (DIM == 1 || b < geo.nblocks);
-        const uint32_t byte = ok ? (uint32_t)cmp[b] : 0u;
-        if (j < NBW) sm.hdr[wave][j] = (uint8_t)byte;
-        wsum += ok ? view_of<MODE, S>(byte).size : 0u;
+        okb[k] = j < NBW && (DIM == 1 || b < geo.nblocks);
+        const uint32_t byte = okb[k] ? (uint32_t)cmp[b] : 0u;
+        v[k] = view_of<MODE, S>(byte);
+        sz[k] = okb[k] ? v[k].size : 0u;
+        lsum += sz[k];
+        r32 |= (uint32_t)(byte >= 32);
     }
-    const uint32_t wtot = rdlane(scan_wave(wsum), kWaveSize - 1);   // < 2^32; a cross-lane scan: all lanes
+    const uint32_t incl = scan_wave(lsum);                 // < 2^32; a cross-lane scan: all lanes
+    const uint32_t wtot = rdlane(incl, kWaveSize - 1);
+    const uint32_t excl0 = incl - lsum;                    // offset of this lane's first block
+    {
+        uint32_t off = excl0;                              // offsets within the wave's payload
+#pragma unroll
+        for (int k = 0; k < BPL; ++k) {
+            const int j = BPL * lane + k;
+            // every entry is written: blocks past the array get an empty lane mask
+            if (j < NBW) sm.meta[wave][j] = make_dmeta<DIM>(off, v[k], okb[k] && sz[k] != 0);
+            off += sz[k];
+        }
+    }
+    // 2D/3D blocks of 32 planes (not in outlier streams) take the general loop
+    bool wide_blocks = false;
+    if constexpr (DIM != 1 && MODE != kOutlier) wide_blocks = ballot(r32 != 0) != 0;
+
     uint64_t wave_base;
     if constexpr (OFFS) {
         (void)sc;
         wave_base = uniform64(offs[lt * WAVES + (uint64_t)wave]);
-        __builtin_amdgcn_wave_barrier();   // the header bytes in LDS are this wave's own
+        __builtin_amdgcn_wave_barrier();   // the metadata in LDS is this wave's own
     } else {
         if (lane == 0) sm.wtot[wave] = wtot;
         __syncthreads();
@@ -1504,15 +1592,37 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     const bool intact = wave_base + wtot <= cur.limit;
     if (!intact && lane == 0) atomicOr(cur.err, kErrTruncated);
     // The wave's stream words go through a buffer resource over its payload:
-    // 32-bit offsets, and an offset past the range (kAbsent) reads 0, so lanes
-    // without a word need no address select and no masking; a truncated stream
-    // gets an empty range (every word reads 0).
+    // 32-bit offsets, and an offset past the range (bit 31 set) reads 0, so
+    // lanes without a word need no address select and no masking; a truncated
+    // stream gets an empty range (every word reads 0).
     const uint64_t span = cur.limit - wave_base;
     const int nrec = __builtin_amdgcn_readfirstlane(intact ? (int)(span < 0x7fffffffull ? span : 0x7fffffffull) : 0);
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)wbase, 0, nrec, kBufferWord3);
 
+    // outlier streams: |d0| prefix word of each selected block of this lane
+    // (masked with PK at the use, so nothing waits for the load here); a
+    // selected block starts with nb <= 4 prefix bytes and then a 4-byte sign
+    // word, so one u32 read at the block start stays inside the block
+    uint32_t PR[BPL], PK[BPL], PV[BPL];   // PV = PR & PK, formed once the first chunk's loads are issued
+#pragma unroll
+    for (int k = 0; k < BPL; ++k) PR[k] = PK[k] = PV[k] = 0;
+    auto prefixes_ready = [&]() {
+#pragma unroll
+        for (int k = 0; k < BPL; ++k) PV[k] = PR[k] & PK[k];
+    };
+    if constexpr (MODE == kOutlier) {
+        uint32_t off = incl - lsum;
+#pragma unroll
+        for (int k = 0; k < BPL; ++k) {
+            const bool pre = okb[k] && v[k].sel;
+            PR[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, pre ? off : kAbsent, 0, 0);
+            PK[k] = v[k].nb < 4 ? (1u << (8 * v[k].nb)) - 1 : ~0u;
+            off += sz[k];
+        }
+    }
+
     XposeConsts xc;
-    xc.init(lane);
+    xc.init(lane, true);
     LaneGeo<DIM> lg;
     lg.init(lane, geo);
     T* __restrict__ out = (T*)cur.dst;
@@ -1524,93 +1634,61 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     // consecutive along x
     const bool fast = groups_full<DIM, kChunks * G>(geo, wgrp0) && (DIM == 1 || geo.Bx % G == 0);
 
-    // Bookkeeping of chunk c (payload offset `coff` within the wave): metadata
-    // into slot s; for outlier streams lane j also loads block j's |d0| prefix
-    // word into PR[s] (masked by PK[s] at the use, so nothing waits for the
-    // load here). Returns the chunk's byte count. Loads that are not needed read
-    // the zero word, so the code has no branches around memory operations.
-    uint32_t PR0 = 0, PR1 = 0, PK0 = 0, PK1 = 0;
-    auto bookkeep = [&](int c, uint32_t coff, int s) -> uint32_t {
-        const bool blk = lane < NB;
-        const uint32_t byte = blk ? (uint32_t)sm.hdr[wave][c * NB + lane] : 0u;
-        const uint64_t b = wb0 + (uint64_t)(c * NB + lane);
-        const bool ok = blk && (DIM == 1 || b < geo.nblocks);
-        const BlockView v = view_of<MODE, S>(byte);
-        const uint32_t sz = ok ? v.size : 0u;
-        const uint32_t incl = scan_wave(sz);
-        const uint32_t off = coff + incl - sz;
-        // every entry is rewritten: blocks past the end get an empty lane mask
-        if (blk) sm.meta[s][wave][lane] = make_meta(off, v, ok && sz != 0);
-        if constexpr (MODE == kOutlier) {
-            // a selected block starts with nb <= 4 prefix bytes and then a 4-byte
-            // sign word, so one u32 read at the block start stays inside the block
-            const bool pre = ok && v.sel;
-            const uint32_t o = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, pre ? off : kAbsent, 0, 0);
-            const uint32_t k = v.nb < 4 ? (1u << (8 * v.nb)) - 1 : ~0u;
-            if (s == 0) {
-                PR0 = o;
-                PK0 = k;
-            } else {
-                PR1 = o;
-                PK1 = k;
-            }
-        }
-        return rdlane(incl, kWaveSize - 1);
+    // metadata of this lane's block in group g of chunk c
+    auto meta = [&](int c, int g) -> DMeta { return sm.meta[wave][c * NB + BPG * g + (DIM == 1 ? (int)h : 0)]; };
+    // stream word of this lane in group g of chunk c (0 for a lane without a
+    // word and for every lane of a truncated stream); no 32-plane blocks
+    auto word = [&](int c, int g) -> uint32_t {
+        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, meta(c, g).word_at(l, wo), 0, 0);
     };
-    // stream word of this lane in group g of the chunk in slot s (0 for a lane
-    // without a word and for every lane of a truncated stream)
-    auto word = [&](int s, int g) -> uint32_t {
-        const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
+    // the same where 2D/3D blocks of 32 planes may occur
+    auto word_any = [&](int c, int g) -> uint32_t {
+        const DMeta m = meta(c, g);
         const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
         const uint32_t o = m.off() + (wideg ? wo_wide : wo);
-        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, m.has(l) ? o : kAbsent, 0, 0);
+        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, ((m.nb >> l) & 1) ? kAbsent : o, 0, 0);
     };
-    // decodes group g of the chunk in slot s from its word after the transpose
-    // residuals (natural element order) of group g of the chunk in slot s from
-    // its word after the transpose
-    auto residual_x = [&](int s, int g, uint32_t x, auto widec) -> uint32_t {
+    // residuals of group g of chunk c from its word after the transpose (lane
+    // e: element e)
+    auto residual_x = [&](int c, int g, uint32_t x, auto widec) -> uint32_t {
         constexpr bool WIDE = decltype(widec)::value;   // 2D/3D 32-plane blocks possible
-        const Meta m = sm.meta[s][wave][BPG * g + (DIM == 1 ? h : 0)];
+        const DMeta m = meta(c, g);
         uint32_t a = x & 0x7fffffffu, neg = x >> 31;   // lane l: sign in bit 31 over the magnitude bits
         if constexpr (WIDE) {
             if (m.wide()) {   // 32 planes: the sign words sit in their own two words
                 const uint32_t sw = (intact && m.nz()) ? load_u32u(wbase + (m.off() + 4 * h)) : 0u;
-                neg = (sw >> l) & 1;
+                neg = (sw >> (l ^ 7)) & 1;   // element l's sign: stream bit l^7
                 a = x;
             }
         }
         if constexpr (MODE == kOutlier) {   // element 0 of a selected block (0 when not intact)
-            const uint32_t pv = s == 0 ? (PR0 & PK0) : (PR1 & PK1);   // lane j: block j's |d0|
+            // block c*NB + BPG*g (+ h in 1D) is held by bookkeeping lane
+            // (block / BPL), slot (block % BPL)
             uint32_t e0;
-            if constexpr (DIM == 1) e0 = h ? rdlane(pv, 2 * g + 1) : rdlane(pv, 2 * g);
-            else e0 = rdlane(pv, g);
-            a = (m.sel() && l == 7 && (DIM == 1 || h == 0)) ? e0 : a;
+            if constexpr (BPL == 2) {
+                static_assert(DIM == 1 && NB == 2 * G, "two blocks per lane: 1D f32");
+                const int ln = c * G + g;
+                e0 = h ? rdlane(PV[BPL - 1], ln) : rdlane(PV[0], ln);
+            } else if constexpr (DIM == 1) {
+                e0 = h ? rdlane(PV[0], c * NB + 2 * g + 1) : rdlane(PV[0], c * NB + 2 * g);
+            } else {
+                e0 = rdlane(PV[0], c * NB + g);
+            }
+            a = (m.sel() && l == 0 && (DIM == 1 || h == 0)) ? e0 : a;
         }
-        const uint32_t dm = neg ? 0u - a : a;
-        return dpp<kDppHalfMirror>(dm);   // back to natural element order
+        return neg ? 0u - a : a;
     };
     // the value of code q (a block without payload, or of a truncated stream,
     // decoded all-zero words: q = 0)
     auto value_of = [&](uint32_t q) -> T { return (T)(int32_t)q * eb * T(2); };   // src/cuSZp_kernels_1D_f32.cu:490
-    auto decode_x = [&](int s, int g, uint32_t x, auto widec) -> T {
-        const uint32_t d = residual_x(s, g, x, widec);
+    auto decode_x = [&](int c, int g, uint32_t x, auto widec) -> T {
+        const uint32_t d = residual_x(c, g, x, widec);
         if constexpr (MODE == kFixed) return value_of(d);
         else return value_of(inverse_lorenzo<DIM>(d, lane));
     };
-    // decodes group g of the chunk in slot s from its loaded word w
-    auto decode = [&](int s, int g, uint32_t w, auto widec) -> T { return decode_x(s, g, xpose32(w, xc), widec); };
+    // decodes group g of chunk c from its loaded word w
+    auto decode = [&](int c, int g, uint32_t w, auto widec) -> T { return decode_x(c, g, xpose32(w, xc), widec); };
 
-    // 2D/3D blocks of 32 planes (not in outlier streams) take the general loop
-    bool wide_blocks = false;
-    if constexpr (DIM != 1 && MODE != kOutlier) {
-        uint32_t r32 = 0;
-#pragma unroll
-        for (int i = 0; i < (NBW + kWaveSize - 1) / kWaveSize; ++i) {
-            const int j = i * kWaveSize + lane;
-            if (j < NBW) r32 |= sm.hdr[wave][j] >= 32;
-        }
-        wide_blocks = ballot(r32 != 0) != 0;
-    }
     if (fast && !wide_blocks) {
         // ---- every element inside the array, chunks along x: no branches ----------
         // Chunks go in pairs through two word buffers: a chunk's loads are
@@ -1654,7 +1732,7 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
                 rdelta = (uint64_t)(64 / VPR) * geo.X;
             }
         }
-        auto decode_chunk = [&](int c, int s, const uint32_t (&W)[G]) {
+        auto decode_chunk = [&](int c, const uint32_t (&W)[G]) {
             uint32_t X[G];
 #pragma unroll
             for (int g = 0; g < G; ++g) X[g] = W[g];
@@ -1663,7 +1741,7 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
             for (int g = 0; g < G; g += XB) xpose32_n<XB>(X + g, xc);
             // X[g]: residuals, then (batch by batch) codes
 #pragma unroll
-            for (int g = 0; g < G; ++g) X[g] = residual_x(s, g, X[g], std::false_type{});
+            for (int g = 0; g < G; ++g) X[g] = residual_x(c, g, X[g], std::false_type{});
             if constexpr (MODE != kFixed) {
 #pragma unroll
                 for (int g = 0; g < G; g += XB) inverse_lorenzo_n<DIM, XB>(X + g, lane);
@@ -1689,52 +1767,116 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
                 wkc.step(geo, G);
             }
         };
-        static_assert(kChunks % 2 == 0, "chunks go in pairs");
-        uint32_t WA[G], WB[G];
-        uint32_t coff = bookkeep(0, 0, 0);
+        // Stream words. A chunk's payload is contiguous: where every chunk of the
+        // wave spans at most kDenseBytes from its 4-byte-aligned start, it is read
+        // with one or two dense 16-byte-per-lane loads into the wave's LDS buffer
+        // and each lane reads its words from there (lanes without a word read a
+        // zero word). Loads with some lanes past the buffer range or 14-of-64
+        // active lanes cost far more memory-pipeline time than dense ones.
+        // Otherwise each lane loads its own word from global memory.
+        uint32_t cs[kChunks + 1];   // chunk starts within the wave's payload (wave-uniform)
 #pragma unroll
-        for (int g = 0; g < G; ++g) WA[g] = word(0, g);
-        for (int c = 0; c + 2 < kChunks; c += 2) {
-            coff += bookkeep(c + 1, coff, 1);
-            __builtin_amdgcn_wave_barrier();
+        for (int c = 0; c < kChunks; ++c) cs[c] = rdlane(excl0, c * NB / BPL);
+        cs[kChunks] = wtot;
+        // Buffer loads check their range per dword, so the dwords holding the
+        // payload's last bytes must lie inside it: an outlier stream whose length
+        // is not a multiple of 4 ends inside a dword, and its last wave keeps the
+        // per-lane loads (a truncated stream has an empty range: the same).
+        bool dense = ((wtot + 3u) & ~3u) <= (uint32_t)nrec;
 #pragma unroll
-            for (int g = 0; g < G; ++g) WB[g] = word(1, g);
-            decode_chunk(c, 0, WA);
-            coff += bookkeep(c + 2, coff, 0);
-            __builtin_amdgcn_wave_barrier();
+        for (int c = 0; c < kChunks; ++c) dense &= cs[c + 1] - (cs[c] & ~3u) <= (uint32_t)kDenseBytes;
+        if (dense) {
+            typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+            lds_u8* const buf = (lds_u8*)sm.ostage[wave];
+            if (lane < 2) sm.zero[wave][lane] = 0;
+            const lds_u8* const zw = (const lds_u8*)&sm.zero[wave][0];
+            // the chunk's payload from its aligned start; the second KiB only when
+            // needed (else an offset past every range: no memory access)
+            auto load_chunk = [&](int c, v4u (&P)[2]) {
+                const uint32_t a0 = cs[c] & ~3u;
+                P[0] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, a0 + 16u * (uint32_t)lane, 0, 0);
+                P[1] = __builtin_amdgcn_raw_buffer_load_b128(
+                    rsrc, cs[c + 1] - a0 > 1024u ? a0 + 1024u + 16u * (uint32_t)lane : kAbsent, 0, 0);
+            };
+            auto stage_chunk = [&](const v4u (&P)[2]) {
+                __builtin_amdgcn_wave_barrier();   // the previous chunk's reads of the buffer come first
+                ((__attribute__((address_space(3))) v4u*)buf)[lane] = P[0];
+                ((__attribute__((address_space(3))) v4u*)buf)[64 + lane] = P[1];
+                __builtin_amdgcn_wave_barrier();
+            };
+            auto word_lds = [&](int c, int g) -> uint32_t {
+                const DMeta m = meta(c, g);
+                typedef __attribute__((address_space(3))) const uint32_t lds_cu32;
+                if constexpr (MODE != kOutlier) {
+                    // fixed/plain blocks are whole 4-byte words: an aligned read
+                    const uint32_t o = m.a + wo - (cs[c] & ~3u);
+                    return *(lds_cu32*)(((m.nb >> l) & 1) ? zw : buf + o);
+                } else {
+                    // outlier prefixes shift later blocks by 1-3 bytes: the two
+                    // aligned words around the word, then a byte funnel shift
+                    const uint32_t o = (m.a & 0xfffffu) + wo - (cs[c] & ~3u);
+                    lds_cu32* const q = (lds_cu32*)(((m.nb >> l) & 1) ? zw : buf + (o & ~3u));
+                    return __builtin_amdgcn_alignbyte(q[1], q[0], o & 3u);
+                }
+            };
+            v4u PA[2], PB[2];
+            load_chunk(0, PA);
+            load_chunk(1, PB);
+            prefixes_ready();
+            static_for<0, kChunks>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                v4u (&P)[2] = (c % 2 == 0) ? PA : PB;
+                stage_chunk(P);
+                if constexpr (c + 2 < kChunks) load_chunk(c + 2, P);
+                uint32_t W[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) W[g] = word_lds(c, g);
+                decode_chunk(c, W);
+            });
+        } else {
+            static_assert(kChunks % 2 == 0, "chunks go in pairs");
+            // Fully unrolled: in a loop, the wait for a chunk's words at the loop
+            // head would be merged with the path from before the loop (where no
+            // stores follow the loads) and so count the previous chunk's stores as
+            // well, putting their completion on the critical path.
+            uint32_t WA[G], WB[G];
 #pragma unroll
             for (int g = 0; g < G; ++g) WA[g] = word(0, g);
-            decode_chunk(c + 1, 1, WB);
-        }
-        // the last pair: nothing more to load after it
-        coff += bookkeep(kChunks - 1, coff, 1);
-        __builtin_amdgcn_wave_barrier();
+            prefixes_ready();
 #pragma unroll
-        for (int g = 0; g < G; ++g) WB[g] = word(1, g);
-        decode_chunk(kChunks - 2, 0, WA);
-        decode_chunk(kChunks - 1, 1, WB);
+            for (int c = 0; c + 2 < kChunks; c += 2) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) WB[g] = word(c + 1, g);
+                decode_chunk(c, WA);
+#pragma unroll
+                for (int g = 0; g < G; ++g) WA[g] = word(c + 2, g);
+                decode_chunk(c + 1, WB);
+            }
+            // the last pair: nothing more to load after it
+#pragma unroll
+            for (int g = 0; g < G; ++g) WB[g] = word(kChunks - 1, g);
+            decode_chunk(kChunks - 2, WA);
+            decode_chunk(kChunks - 1, WB);
+        }
     } else {
         // ---- general: per-element checks, 32-plane blocks -------------------------
         Walker wk;
         if constexpr (DIM != 1) wk.init(wgrp0, geo);
-        uint32_t coff = 0;
         const uint64_t gvalid = (DIM == 1) ? (geo.n + 63) / 64 : geo.nblocks;   // groups holding elements
+        prefixes_ready();
         for (int c = 0; c < kChunks; ++c) {
             if (wgrp0 + (uint64_t)c * G >= gvalid) break;   // the rest of the wave writes nothing
-            coff += bookkeep(c, coff, 0);
-            __builtin_amdgcn_wave_barrier();
             uint32_t W[G];
 #pragma unroll
-            for (int g = 0; g < G; ++g) W[g] = word(0, g);
+            for (int g = 0; g < G; ++g) W[g] = word_any(c, g);
             const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const T val = decode(0, g, W[g], std::true_type{});
+                const T val = decode(c, g, W[g], std::true_type{});
                 uint64_t idx;
                 if (elem_of<DIM>(geo, lg, wk, grp0 + g, lane, idx)) out[idx] = val;
                 if constexpr (DIM != 1) wk.next(geo);
             }
-            __builtin_amdgcn_wave_barrier();
         }
     }
 }
