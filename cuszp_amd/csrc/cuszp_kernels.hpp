@@ -238,6 +238,20 @@ __device__ __forceinline__ uint64_t stream_order64(uint64_t m) {
     return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
+// p[lane - 1] - q[lane] in one DPP subtract (wave_shr:1; lane 0 reads p = 0).
+// Measured on gfx950 with this assembler: v_sub_u32_dpp d, a, b gives
+// a[lane - 1] - b[lane]; v_subrev_u32_dpp d, a, b gives b[lane - 1] - a[lane]
+// (the shift is on its second operand), so neither forms q[lane] - q[lane - 1]
+// in one instruction. The two wait states a DPP read of a VGPR needs after the
+// VALU write of it are in the block, since the compiler cannot see into it.
+__device__ __forceinline__ uint32_t sub_prev_wave_shr1(uint32_t p, uint32_t q) {
+    uint32_t d;
+    asm("s_nop 1\n\tv_sub_u32_dpp %0, %1, %2 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "=v"(d)
+        : "v"(p), "v"(q));
+    return d;
+}
+
 // Inclusive prefix sums (wrapping u32).
 __device__ __forceinline__ uint32_t scan_row16(uint32_t v) {
     v += dpp<kDppRowShr1>(v);
@@ -676,32 +690,30 @@ __device__ __forceinline__ void inverse_lorenzo_n(uint32_t* d, int e) {
 
 // ---------------------------------------------------------------------------
 // per-block metadata handed from the bookkeeping lanes to the stream-word
-// lanes through LDS (one 8-byte entry per block):
-//   a: [19:0] byte offset of the block's sign word from the wave's first
-//      block (past any outlier prefix), [30] 32 planes (2D/3D: the sign words
-//      move in front of the planes), [31] outlier-selected
-//   b: bit l set when lane l of a 32-lane half carries one of the block's
-//      stream words (planes 0..R-1 in lanes 0..R-1, the sign word in lane 31)
+// lanes through LDS (one 16-byte entry per block):
+//   a: byte offset of the block's sign word from the wave's first block (past
+//      any outlier prefix)
+//   b: 0 for a block without payload, else R + 1 (R planes; 33 for a 2D/3D
+//      block of 32 planes). Lane l of a 32-lane half carries one of the
+//      block's words iff key(l) < b, with key(31) = 0 (the sign word, or plane
+//      31 of a 32-plane block) and key(l) = l + 1 (plane l): one compare.
+//   c: the plane words' mask of an outlier-selected block (element 0 travels
+//      in the prefix: bit 7, element 0's stream bit, cleared), else all ones
 // ---------------------------------------------------------------------------
 struct Meta {
-    uint32_t a, b;
-    __device__ __forceinline__ uint32_t off() const { return a & 0xfffffu; }
-    __device__ __forceinline__ bool wide() const { return (a >> 30) & 1; }
-    __device__ __forceinline__ bool sel() const { return (int32_t)a < 0; }
+    uint32_t a, b, c, pad;
+    __device__ __forceinline__ uint32_t off() const { return a; }
+    __device__ __forceinline__ bool sel() const { return c != ~0u; }
     __device__ __forceinline__ bool nz() const { return b != 0; }
-    __device__ __forceinline__ bool has(uint32_t l) const { return (b >> l) & 1; }
-    // lane l's word offset o if the lane carries a word, else `absent`: the
-    // lane's bit sign-extended to a mask (v_bfe_i32) selects with one v_bfi
-    __device__ __forceinline__ uint32_t pick(uint32_t l, uint32_t o, uint32_t absent) const;
+    __device__ __forceinline__ bool has(uint32_t key) const { return key < b; }
 };
-__device__ __forceinline__ uint32_t Meta::pick(uint32_t l, uint32_t o, uint32_t absent) const {
-    return bfi((uint32_t)__builtin_amdgcn_sbfe((int)b, (int)l, 1), o, absent);
-}
+template <int MODE>
 __device__ __forceinline__ Meta make_meta(uint32_t off, const BlockView& v, bool nz) {
     Meta m;
-    const bool wide = v.R >= 32;
-    m.a = (off + v.head) | ((uint32_t)wide << 30) | (v.sel << 31);
-    m.b = !nz ? 0u : wide ? ~0u : (((1u << v.R) - 1u) | 0x80000000u);
+    m.a = off + v.head;
+    m.b = nz ? v.R + 1u : 0u;
+    m.c = (MODE == kOutlier && v.sel) ? ~0x80u : ~0u;
+    m.pad = 0;
     return m;
 }
 
@@ -1001,8 +1013,8 @@ struct SmemC {
     uint64_t wtot[WAVES];                      // per-wave byte counts
     uint64_t excl;                             // the tile's exclusive prefix (wave 0's lookback)
     uint32_t over[WAVES];                      // the wave's payload did not fit its staging area
-    Meta meta[WAVES][32];                      // per-block metadata of the chunk in work
-    uint32_t aux[WAVES][32];                   // sign words of the chunk's 32-plane 2D/3D groups
+    Meta meta[WAVES][16];                      // per-block metadata of the chunk in work
+    uint32_t aux[WAVES][16];                   // sign words of the chunk's 32-plane 2D/3D groups
     uint8_t hdr[WAVES][128];                   // header bytes of the wave's blocks, stored after its chunks
     uint4 stage[WAVES][kStageBytes / 16 + 1];  // the wave's payload, as it will appear in the stream (+ a spare word)
 };
@@ -1118,10 +1130,10 @@ __device__ __forceinline__ bool groups_full(const Geom& geo, uint64_t grp0) {
 // Each wave runs its kChunks chunks of G groups in turn:
 //   1. loads (one element per lane per group; a chunk's G loads are issued
 //      before the previous chunk is processed);
-//   2. pass 1, per group: quantise, Lorenzo residual, mirror, sign|magnitude,
-//      bit transpose; nonzero-plane masks go to the block's bookkeeping lane;
-//   3. bookkeeping, lane j <-> block j of the chunk: header byte (stored at
-//      once, its position is fixed), size, offset; metadata through LDS;
+//   2. pass 1, per group: quantise, Lorenzo residual, sign|magnitude, bit
+//      transpose; nonzero-plane masks go to the block's bookkeeping lane;
+//   3. bookkeeping, lane j <-> block j of the chunk: header byte (to LDS, stored
+//      after the chunks), size, offset; metadata through LDS;
 //   4. the chunk's stream words are written into the wave's LDS staging area
 //      exactly as they will appear in the stream.
 // The wave totals then combine in LDS, wave 0 runs the lookback (publishing
@@ -1136,7 +1148,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
     constexpr int S = (DIM == 1) ? 4 : 8;        // bytes per plane
     constexpr int BPG = (DIM == 1) ? 2 : 1;      // blocks per group
     constexpr int NB = BPG * G;                  // blocks per chunk
-    static_assert(NB <= 32 && 2 * G <= 32, "shared metadata sized for 32 blocks per chunk");
+    static_assert(NB <= 16 && 2 * G <= 16, "shared metadata sized for 16 blocks per chunk");
 
     const int lane = (int)__lane_id();
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1145,6 +1157,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
     const T r = (T)cur.param;
     const uint64_t lt = tile - cur.tile0;
     uint8_t* __restrict__ const dst = (uint8_t*)cur.dst;
+    const T* __restrict__ const in = (const T*)cur.src;
     const uint64_t wgrp0 = (lt * WAVES + (uint64_t)wave) * (kChunks * G);   // this wave's first group
 
     LaneGeo<DIM> lg;
@@ -1156,340 +1169,334 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
     // after the transpose, element 0 of a block is bit 7 of its plane words: in
     // both halves for 1D (two blocks), only in the low half for 2D/3D
     const uint32_t not_e0 = (DIM == 1 || h == 0) ? ~0x80u : ~0u;
+    // bit 7 of this lane's word is not element 0 of a plane (the sign word, or
+    // the high half of a 2D/3D block): kept in selected blocks too
+    const uint32_t keep_e0 = (l == 31 || (DIM != 1 && h != 0)) ? 0x80u : 0u;
+    const uint32_t key = (l == 31) ? 0u : l + 1u;             // see Meta::b
     const uint32_t wo = word_offset<DIM>(l, h, true);        // this lane's word in a block's payload
     const uint32_t wo_wide = word_offset<DIM>(l, h, false);  // same, 32-plane 2D/3D layout
+    // LDS addresses of this lane's word in the staging area (offset 0) and of
+    // the spare word past it, where lanes without a word write
+    lds_u8* const sbase = (lds_u8*)sm.stage[wave];
+    const uint32_t s0 = (uint32_t)(size_t)sbase;
+    const uint32_t so = s0 + wo, so_wide = s0 + wo_wide, spare = s0 + kStageBytes;
 
     uint64_t excl = 0, wexcl = 0, ttot = 0;
     uint32_t wave_tot = 0;
     bool over = false;
     bool lossy = false;   // a 2D/3D outlier block got rate byte 32 (see kWarnLossy)
-    for (int pass = 0;; ++pass) {
-        // pass 0: header bytes and staging; pass 1 (a wave that did not fit):
-        // direct stores at gbase
-        const bool direct = pass == 1;
-        uint8_t* const gbase = dst + uniform64(geo.H + excl + wexcl);
-        wave_tot = 0;
-        const T* __restrict__ const in = (const T*)cur.src;
-        // Processes chunk c from xv (vmask: valid elements per group); roll(g)
-        // runs right after group g's value is consumed (the fast path loads the
-        // next chunk's group g into xv[g] there).
-        auto process = [&](int c, T (&xv)[G], const uint64_t vmask, auto&& roll) {
-            const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
 
-            // ---- pass 1 ---------------------------------------------------------
-            // WIDE = false: the branch-free form, which assumes no 2D/3D block
-            // holds a magnitude of 2^31 and reports the groups where one does;
-            // the chunk is then rerun from memory with WIDE = true.
-            uint32_t W[G];
-            uint32_t bk1, bk2, bko;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
-            uint32_t big;             // bit g: group g's block has a magnitude 2^31 (2D/3D)
-            auto pass1 = [&](auto widec, auto&& xval, const uint64_t vm, auto&& roll_fn) -> uint64_t {
-                constexpr bool WIDE = decltype(widec)::value;
-                bk1 = bk2 = bko = big = 0;
-                uint64_t anywide = 0;
-                if constexpr (!WIDE) {
-                    // the words of all groups first, then their transposes side by
-                    // side (one LDS round trip per stage for kXposeBatch groups),
-                    // then the plane masks
-                    static_for<0, G>([&](auto gc) {
-                        constexpr int g = decltype(gc)::value;
-                        const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array (q = -1 if r = inf: masked below)
-                        roll_fn(g);
-                        uint32_t d;
-                        if constexpr (MODE == kFixed) {
-                            d = q;
-                        } else if constexpr (DIM == 1) {
-                            uint32_t qp = dpp<kDppWaveShr1>(q);
-                            qp = (lane & 31) ? qp : 0u;
-                            d = q - qp;
-                        } else {
-                            uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
-                            qp = (pl >= 0) ? qp : 0u;
-                            d = q - qp;
-                        }
-                        d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
-                        const uint32_t a = ((int32_t)d < 0) ? 0u - d : d;
-                        if constexpr (DIM != 1) anywide |= ballot((int32_t)a < 0);
-                        // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
-                        // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
-                        W[g] = bfi(0x80000000u, d, a);
-                        if constexpr (MODE == kOutlier) {   // |d| of each block's element 0
-                            if constexpr (DIM == 1) {
-                                bko = wrlane<2 * g>(rdlane(a, 0), bko);
-                                bko = wrlane<2 * g + 1>(rdlane(a, 32), bko);
-                            } else {
-                                bko = wrlane<g>(rdlane(a, 0), bko);
-                            }
-                        }
-                    });
-                    constexpr int XB = kXposeBatch < G ? kXposeBatch : G;
-#pragma unroll
-                    for (int g = 0; g < G; g += XB) xpose32_n<XB>(W + g, xc);
-                    static_for<0, G>([&](auto gc) {
-                        constexpr int g = decltype(gc)::value;
-                        const uint64_t m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
-                        uint64_t m2 = 0;
-                        if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
-                        if constexpr (DIM == 1) {
-                            bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
-                            bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
-                            if constexpr (MODE == kOutlier) {
-                                bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
-                                bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
-                            }
-                        } else {
-                            bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
-                            if constexpr (MODE == kOutlier) bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
-                        }
-                    });
-                    return anywide;
-                }
+    // Processes chunk c from xv (vmask: valid elements per group); roll(g) runs
+    // right after group g's value is consumed (the fast path loads the next
+    // chunk's group g into xv[g] there). DIRECT: store the words at gbase
+    // instead of staging them (and write no header bytes).
+    auto process = [&](auto directc, uint8_t* const gbase, int c, T (&xv)[G], const uint64_t vmask, auto&& roll) {
+        constexpr bool DIRECT = decltype(directc)::value;
+        const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
+
+        // ---- pass 1 ---------------------------------------------------------
+        // WIDE = false: the branch-free form, which assumes no 2D/3D block
+        // holds a magnitude of 2^31 and reports the groups where one does;
+        // the chunk is then rerun from memory with WIDE = true.
+        uint32_t W[G];
+        uint32_t bk1, bk2, bko;   // lane j: block j's nonzero-plane mask (all / e >= 1), |d| of element 0
+        uint32_t big;             // bit g: group g's block has a magnitude 2^31 (2D/3D)
+        auto residual = [&](const uint32_t q) -> uint32_t {
+            if constexpr (MODE == kFixed) {
+                return q;
+            } else if constexpr (DIM == 1) {
+                // q - q[lane - 1], negated from one DPP subtract (the negation is
+                // the subtraction |d| needs anyway); element 0 of each block has no
+                // predecessor: lane 31's code is not passed on, lane 0 reads 0
+                const uint32_t p = ((lane & 31) == 31) ? 0u : q;
+                return 0u - sub_prev_wave_shr1(p, q);
+            } else {
+                uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
+                qp = (pl >= 0) ? qp : 0u;
+                return q - qp;
+            }
+        };
+        auto pass1 = [&](auto widec, auto&& xval, const uint64_t vm, auto&& roll_fn) -> uint64_t {
+            constexpr bool WIDE = decltype(widec)::value;
+            bk1 = bk2 = bko = big = 0;
+            uint64_t anywide = 0;
+            if constexpr (!WIDE) {
+                // the words of all groups first, then their transposes side by
+                // side (one LDS round trip per stage for kXposeBatch groups),
+                // then the plane masks
                 static_for<0, G>([&](auto gc) {
                     constexpr int g = decltype(gc)::value;
                     const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array (q = -1 if r = inf: masked below)
                     roll_fn(g);
-                    uint32_t d;
-                    if constexpr (MODE == kFixed) {
-                        d = q;
-                    } else if constexpr (DIM == 1) {
-                        uint32_t qp = dpp<kDppWaveShr1>(q);
-                        qp = (lane & 31) ? qp : 0u;
-                        d = q - qp;
-                    } else {
-                        uint32_t qp = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)q);
-                        qp = (pl >= 0) ? qp : 0u;
-                        d = q - qp;
-                    }
+                    uint32_t d = residual(q);
                     d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
                     const uint32_t a = ((int32_t)d < 0) ? 0u - d : d;
-                    uint64_t m1, m2 = 0;
-                    bool wide = false;
-                    if constexpr (DIM != 1) {
-                        if constexpr (WIDE) wide = ballot((int32_t)a < 0) != 0;
-                        else anywide |= ballot((int32_t)a < 0);
-                    }
-                    if (!wide) {
-                        // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
-                        // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
-                        W[g] = xpose32(bfi(0x80000000u, d, a), xc);
-                        m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
-                        if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
-                    } else {
-                        W[g] = xpose32(a, xc);
-                        m1 = ballot(W[g] != 0);
-                        if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0);
-                        const uint64_t sg = stream_order64(ballot((int32_t)d < 0));   // the sign words
-                        if (lane == 0) {
-                            sm.aux[wave][2 * g] = (uint32_t)sg;
-                            sm.aux[wave][2 * g + 1] = (uint32_t)(sg >> 32);
+                    if constexpr (DIM != 1) anywide |= ballot((int32_t)a < 0);
+                    // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
+                    // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
+                    W[g] = bfi(0x80000000u, d, a);
+                    if constexpr (MODE == kOutlier) {   // |d| of each block's element 0
+                        if constexpr (DIM == 1) {
+                            bko = wrlane<2 * g>(rdlane(a, 0), bko);
+                            bko = wrlane<2 * g + 1>(rdlane(a, 32), bko);
+                        } else {
+                            bko = wrlane<g>(rdlane(a, 0), bko);
                         }
-                        big |= 1u << g;
                     }
+                });
+                constexpr int XB = kXposeBatch < G ? kXposeBatch : G;
+#pragma unroll
+                for (int g = 0; g < G; g += XB) xpose32_n<XB>(W + g, xc);
+                static_for<0, G>([&](auto gc) {
+                    constexpr int g = decltype(gc)::value;
+                    const uint64_t m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
+                    uint64_t m2 = 0;
+                    if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
                     if constexpr (DIM == 1) {
                         bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
                         bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
                         if constexpr (MODE == kOutlier) {
                             bk2 = wrlane<2 * g>((uint32_t)m2, bk2);
                             bk2 = wrlane<2 * g + 1>((uint32_t)(m2 >> 32), bk2);
-                            bko = wrlane<2 * g>(rdlane(a, 0), bko);
-                            bko = wrlane<2 * g + 1>(rdlane(a, 32), bko);
                         }
                     } else {
                         bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
-                        if constexpr (MODE == kOutlier) {
-                            bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
-                            bko = wrlane<g>(rdlane(a, 0), bko);
-                        }
+                        if constexpr (MODE == kOutlier) bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
                     }
                 });
                 return anywide;
-            };
-            const uint64_t anywide = pass1(std::false_type{}, [&](int g) -> T { return xv[g]; }, vmask, roll);
-            if constexpr (DIM != 1) {
-                if (anywide) {   // rare: rerun the chunk with 32-plane blocks
-                    T xr[G];
-                    uint64_t vr;
-                    GroupLoads<DIM, T, G>::template run<false>(in, geo, lg, grp0, lane, xr, vr);
-                    pass1(std::true_type{}, [&](int g) -> T { return xr[g]; }, vr, [](int) {});
+            }
+            static_for<0, G>([&](auto gc) {
+                constexpr int g = decltype(gc)::value;
+                const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array (q = -1 if r = inf: masked below)
+                roll_fn(g);
+                uint32_t d = residual(q);
+                d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
+                const uint32_t a = ((int32_t)d < 0) ? 0u - d : d;
+                uint64_t m1, m2 = 0;
+                const bool wide = ballot((int32_t)a < 0) != 0;
+                if (!wide) {
+                    W[g] = xpose32(bfi(0x80000000u, d, a), xc);
+                    m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
+                    if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
+                } else {
+                    W[g] = xpose32(a, xc);
+                    m1 = ballot(W[g] != 0);
+                    if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0);
+                    const uint64_t sg = stream_order64(ballot((int32_t)d < 0));   // the sign words
+                    if (lane == 0) {
+                        sm.aux[wave][2 * g] = (uint32_t)sg;
+                        sm.aux[wave][2 * g + 1] = (uint32_t)(sg >> 32);
+                    }
+                    big |= 1u << g;
                 }
+                bk1 = wrlane<g>((uint32_t)m1 | (uint32_t)(m1 >> 32), bk1);
+                if constexpr (MODE == kOutlier) {
+                    bk2 = wrlane<g>((uint32_t)m2 | (uint32_t)(m2 >> 32), bk2);
+                    bko = wrlane<g>(rdlane(a, 0), bko);
+                }
+            });
+            return anywide;
+        };
+        const uint64_t anywide = pass1(std::false_type{}, [&](int g) -> T { return xv[g]; }, vmask, roll);
+        if constexpr (DIM != 1) {
+            if (anywide) {   // rare: rerun the chunk with 32-plane blocks
+                T xr[G];
+                uint64_t vr;
+                GroupLoads<DIM, T, G>::template run<false>(in, geo, lg, grp0, lane, xr, vr);
+                pass1(std::true_type{}, [&](int g) -> T { return xr[g]; }, vr, [](int) {});
             }
+        }
 
-            // ---- bookkeeping: lane j <-> block j of this chunk ---------------------
-            const bool blk = lane < NB;
-            uint32_t byte;
-            if constexpr (MODE == kOutlier) byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
-            else byte = bitlen(bk1);
-            const BlockView v = view_of<MODE, S>(byte);
-            if constexpr (MODE == kOutlier && DIM != 1) {
-                // R1 = 32 (a residual of magnitude 2^31, only from saturated or
-                // wrapped codes) is kept as the plain byte 32, which every
-                // outlier decoder reads as rate 0 (written by
-                // src/cuSZp_kernels_2D_f32.cu:1494-1505, masked with 0x1f by the
-                // decoder at :2320): the block decodes to zeros, so the call
-                // reports it
-                lossy |= ballot(blk && byte == 32u) != 0;
-            }
-            const uint32_t sz = blk ? v.size : 0u;
-            const uint32_t incl = scan_wave(sz);
-            const uint32_t ctot = rdlane(incl, kWaveSize - 1);
-            const uint32_t off = wave_tot + incl - sz;   // block offset within the wave's payload
+        // ---- bookkeeping: lane j <-> block j of this chunk ---------------------
+        const bool blk = lane < NB;
+        uint32_t byte;
+        if constexpr (MODE == kOutlier) byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
+        else byte = bitlen(bk1);
+        const BlockView v = view_of<MODE, S>(byte);
+        if constexpr (MODE == kOutlier && DIM != 1) {
+            // R1 = 32 (a residual of magnitude 2^31, only from saturated or
+            // wrapped codes) is kept as the plain byte 32, which every
+            // outlier decoder reads as rate 0 (written by
+            // src/cuSZp_kernels_2D_f32.cu:1494-1505, masked with 0x1f by the
+            // decoder at :2320): the block decodes to zeros, so the call
+            // reports it
+            lossy |= ballot(blk && byte == 32u) != 0;
+        }
+        const uint32_t sz = blk ? v.size : 0u;
+        const uint32_t incl = scan_row16(sz);   // the chunk's NB <= 16 blocks sit in lanes 0..NB-1
+        const uint32_t ctot = rdlane(incl, NB - 1);
+        const uint32_t off = wave_tot + incl - sz;   // block offset within the wave's payload
+        const Meta* const mrow = &sm.meta[wave][DIM == 1 ? h : 0];
+        if constexpr (!DIRECT) {
             // header bytes (lane j: block j) go to the stream after the wave's
             // chunks: no global store inside the chunk loop, whose waits for
             // the rolling loads would otherwise also wait for those stores
-            if (!direct && blk) sm.hdr[wave][c * NB + lane] = (uint8_t)byte;
-            const bool put = direct || (!over && wave_tot + ctot <= kStageBytes);
+            if (blk) sm.hdr[wave][c * NB + lane] = (uint8_t)byte;
+            const bool put = !over && wave_tot + ctot <= kStageBytes;
             over |= !put;
             if (put) {
-                if (blk) sm.meta[wave][lane] = make_meta(off, v, sz != 0);
-                const Meta* const mrow = &sm.meta[wave][DIM == 1 ? h : 0];
-                if (!direct) {
-                    // staging: every lane writes every group (lanes without a word
-                    // of the block write into the spare word past the area), so
-                    // the metadata reads need no branches
-                    lds_u8* const sbase = (lds_u8*)sm.stage[wave];
-                    if constexpr (MODE == kOutlier) {
-                        // |d| of element 0 (4 bytes: the sign word written next
-                        // covers the bytes past the prefix)
-                        if (blk && v.sel) *(lds_u32u*)(sbase + off) = bko;
-                    }
+                if (blk) sm.meta[wave][lane] = make_meta<MODE>(off, v, sz != 0);
+                if constexpr (MODE == kOutlier) {
+                    // |d| of element 0 (4 bytes: the sign word written next
+                    // covers the bytes past the prefix)
+                    if (blk && v.sel) *(lds_u32u*)(sbase + off) = bko;
+                }
+                // staging: every lane writes every group (lanes without a word
+                // of the block write the spare word past the area), so the
+                // metadata reads need no branches
+                if (DIM == 1 || big == 0) {
                     Meta mnext = mrow[0];
                     static_for<0, G>([&](auto gc) {
                         constexpr int g = decltype(gc)::value;
                         const Meta m = mnext;
                         if constexpr (g + 1 < G) mnext = mrow[BPG * (g + 1)];   // one group ahead
-                        const bool wideg = (DIM != 1) && ((big >> g) & 1);
                         uint32_t w = W[g];
-                        if constexpr (MODE == kOutlier) {
-                            if (m.sel() && l != 31) w &= not_e0;   // element 0 travels in the prefix, not the planes
-                        }
-                        const uint32_t o = m.pick(l, m.off() + (wideg ? wo_wide : wo), kStageBytes);
-                        *(lds_u32u*)(sbase + o) = w;
-                        if constexpr (DIM != 1) {
-                            if (wideg && m.nz() && lane < 2) *(lds_u32u*)(sbase + m.off() + 4 * lane) = sm.aux[wave][2 * g + lane];
-                        }
+                        if constexpr (MODE == kOutlier) w &= m.c | keep_e0;   // element 0 travels in the prefix
+                        const uint32_t o = m.has(key) ? m.off() + so : spare;
+                        *(lds_u32u*)(size_t)o = w;
                     });
-                } else {
-                    if constexpr (MODE == kOutlier) {
-                        if (blk && v.sel) store_le(gbase + off, bko, v.nb);   // |d| of element 0, little-endian
-                    }
+                } else {   // a 2D/3D group of 32 planes in the chunk
                     static_for<0, G>([&](auto gc) {
                         constexpr int g = decltype(gc)::value;
-                        const Meta m = mrow[BPG * g];
-                        const bool wideg = (DIM != 1) && ((big >> g) & 1);
+                        const Meta m = mrow[g];
+                        const bool wideg = (big >> g) & 1;
                         uint32_t w = W[g];
-                        if constexpr (MODE == kOutlier) {
-                            if (m.sel() && l != 31) w &= not_e0;
-                        }
-                        if (m.has(l)) store_u32u(gbase + (m.off() + (wideg ? wo_wide : wo)), w);
-                        if constexpr (DIM != 1) {
-                            if (wideg && m.nz() && lane < 2) store_u32u(gbase + (m.off() + 4 * lane), sm.aux[wave][2 * g + lane]);
-                        }
+                        if constexpr (MODE == kOutlier) w &= m.c | keep_e0;
+                        const uint32_t o = m.has(key) ? m.off() + (wideg ? so_wide : so) : spare;
+                        *(lds_u32u*)(size_t)o = w;
+                        if (wideg && m.nz() && lane < 2) *(lds_u32u*)(sbase + m.off() + 4 * lane) = sm.aux[wave][2 * g + lane];
                     });
                 }
             }
-            wave_tot += ctot;
-        };
-        // first element of chunk c's first group for this lane; with a fast
-        // tile, group g of the chunk starts `step` * g elements later
-        constexpr uint64_t step = (DIM == 1) ? 64 : (DIM == 2 ? 8 : 4);
-        // fast tile: every element inside the array and (2D/3D) each chunk's
-        // blocks consecutive along x
-        const bool fast = groups_full<DIM, kChunks * G>(geo, wgrp0) && (DIM == 1 || geo.Bx % G == 0);
-        if (fast) {
-            // wkn: the chunk whose loads are issued next (2D/3D; a run of G blocks
-            // along x, so the walker advances by G)
-            Walker wkn;
-            if constexpr (DIM != 1) wkn.init(wgrp0, geo);
-            auto chunk_ptr = [&](int c) -> const T* {
-                if constexpr (DIM == 1) return in + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
-                else return in + block_origin<DIM>(geo, wkn) + lg.off;
-            };
-            T xv[G];
-            const T* p = chunk_ptr(0);
-#pragma unroll
-            for (int g = 0; g < G; ++g) xv[g] = load_elem<DIM>(p + step * g);
-            for (int c = 0; c + 1 < kChunks; ++c) {
-                // group g of the next chunk loads as soon as group g of this one is coded
-                if constexpr (DIM != 1) wkn.step(geo, G);
-                const T* const pn = chunk_ptr(c + 1);
-                process(c, xv, ~0ull, [&](int g) { xv[g] = load_elem<DIM>(pn + step * g); });
-            }
-            process(kChunks - 1, xv, ~0ull, [](int) {});   // the last chunk loads nothing more
         } else {
-            // groups holding elements of the array; a chunk past them is all
-            // zero: rate bytes 0 in every mode and no payload, so it only writes
-            // its header bytes (1D header padding, 2D/3D pad bytes below H)
-            const uint64_t gvalid = (DIM == 1) ? (geo.n + 63) / 64 : geo.nblocks;
-            for (int c = 0; c < kChunks; ++c) {
-                const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
-                if (grp0 >= gvalid) {
-                    if (!direct && lane < NB) sm.hdr[wave][c * NB + lane] = 0;
-                    continue;
+            if (blk) sm.meta[wave][lane] = make_meta<MODE>(off, v, sz != 0);
+            if constexpr (MODE == kOutlier) {
+                if (blk && v.sel) store_le(gbase + off, bko, v.nb);   // |d| of element 0, little-endian
+            }
+            static_for<0, G>([&](auto gc) {
+                constexpr int g = decltype(gc)::value;
+                const Meta m = mrow[BPG * g];
+                const bool wideg = (DIM != 1) && ((big >> g) & 1);
+                uint32_t w = W[g];
+                if constexpr (MODE == kOutlier) w &= m.c | keep_e0;
+                if (m.has(key)) store_u32u(gbase + (m.off() + (wideg ? wo_wide : wo)), w);
+                if constexpr (DIM != 1) {
+                    if (wideg && m.nz() && lane < 2) store_u32u(gbase + (m.off() + 4 * lane), sm.aux[wave][2 * g + lane]);
                 }
-                T xv[G];
-                uint64_t vmask;
-                GroupLoads<DIM, T, G>::template run<false>(in, geo, lg, grp0, lane, xv, vmask);
-                process(c, xv, vmask, [](int) {});
-            }
-        }
-        if (direct) return;
-
-        // ---- the wave's header bytes (their positions do not depend on the
-        // offset); 2D/3D pad bytes [nblocks, H) are 0 -------------------------------
-        {
-            constexpr int NBW = kChunks * NB;
-            constexpr int BPL = (NBW + kWaveSize - 1) / kWaveSize;
-            static_assert(NBW <= 128, "header bytes of a wave");
-            __builtin_amdgcn_wave_barrier();
-            const uint64_t wb0 = (DIM == 1) ? 2 * wgrp0 : wgrp0;
-#pragma unroll
-            for (int k = 0; k < BPL; ++k) {
-                const int j = BPL * lane + k;
-                const uint64_t b = wb0 + (uint64_t)j;
-                if (j < NBW && (DIM == 1 || b < geo.H)) dst[b] = sm.hdr[wave][j];
-            }
-        }
-
-        // ---- the tile's offset (wave 0) ---------------------------------------------
-        if (lane == 0) {
-            sm.wtot[wave] = wave_tot;
-            sm.over[wave] = over;
-            if (lossy) atomicOr(cur.err, kWarnLossy);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int w = 0; w < WAVES; ++w) {
-            const uint64_t t = sm.wtot[w];
-            wexcl += (w < wave) ? t : 0;
-            ttot += t;
-        }
-        if (wave == 0) {
-            const uint64_t e = wave_lookback<LB>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
-                return compress_tile_bytes<DIM, MODE, T, kChunks * G, WAVES>((const T*)cur.src, geo, r,
-                                                                            p - cur.tile0, lane);
             });
-            if (lane == 0) {
-                sm.excl = e;
-                if (lt == geo.ntiles - 1 && cur.total) *cur.total = geo.H + e + ttot;
+        }
+        wave_tot += ctot;
+    };
+    // The chunks one by one with bounds-checked loads: tiles with elements
+    // past the array, and the direct pass. A chunk past the array's groups is
+    // all zero: rate bytes 0 in every mode and no payload, so it only writes
+    // its header bytes (1D header padding, 2D/3D pad bytes below H).
+    auto checked_chunks = [&](auto directc, uint8_t* const gbase) {
+        constexpr bool DIRECT = decltype(directc)::value;
+        const uint64_t gvalid = (DIM == 1) ? (geo.n + 63) / 64 : geo.nblocks;
+        for (int c = 0; c < kChunks; ++c) {
+            const uint64_t grp0 = wgrp0 + (uint64_t)c * G;
+            if (grp0 >= gvalid) {
+                if (!DIRECT && lane < NB) sm.hdr[wave][c * NB + lane] = 0;
+                continue;
             }
+            T xv[G];
+            uint64_t vmask;
+            GroupLoads<DIM, T, G>::template run<false>(in, geo, lg, grp0, lane, xv, vmask);
+            process(directc, gbase, c, xv, vmask, [](int) {});
         }
-        __syncthreads();
-        excl = sm.excl;
-        if (geo.H + excl + ttot > cur.limit) {   // never write past the caller's buffer
-            if (wave == 0 && lane == 0) atomicOr(cur.err, kErrOverflow);
-            return;
-        }
-        if (over) continue;   // run the chunks again, storing directly
+    };
 
-        // ---- copy the staged payload: 16 bytes per lane, then the tail bytes ---------
-        uint8_t* const obase = dst + uniform64(geo.H + excl + wexcl);
-        typedef uint4 __attribute__((aligned(1))) u128u;
-        const uint32_t n16 = wave_tot & ~15u;
-        for (uint32_t i = 16u * (uint32_t)lane; i < n16; i += 16u * kWaveSize)
-            *(u128u*)(obase + i) = sm.stage[wave][i / 16];
-        if ((uint32_t)lane < (wave_tot & 15u)) obase[n16 + lane] = ((const uint8_t*)sm.stage[wave])[n16 + lane];
+    // ---- pass 0: header bytes and staging ------------------------------------
+    // first element of chunk c's first group for this lane; with a fast tile,
+    // group g of the chunk starts `step` * g elements later
+    constexpr uint64_t step = (DIM == 1) ? 64 : (DIM == 2 ? 8 : 4);
+    // fast tile: every element inside the array and (2D/3D) each chunk's
+    // blocks consecutive along x
+    const bool fast = groups_full<DIM, kChunks * G>(geo, wgrp0) && (DIM == 1 || geo.Bx % G == 0);
+    if (fast) {
+        // wkn: the chunk whose loads are issued next (2D/3D; a run of G blocks
+        // along x, so the walker advances by G)
+        Walker wkn;
+        if constexpr (DIM != 1) wkn.init(wgrp0, geo);
+        auto chunk_ptr = [&](int c) -> const T* {
+            if constexpr (DIM == 1) return in + (wgrp0 + (uint64_t)c * G) * 64 + (uint64_t)lane;
+            else return in + block_origin<DIM>(geo, wkn) + lg.off;
+        };
+        T xv[G];
+        const T* p = chunk_ptr(0);
+#pragma unroll
+        for (int g = 0; g < G; ++g) xv[g] = load_elem<DIM>(p + step * g);
+        for (int c = 0; c + 1 < kChunks; ++c) {
+            // group g of the next chunk loads as soon as group g of this one is coded
+            if constexpr (DIM != 1) wkn.step(geo, G);
+            const T* const pn = chunk_ptr(c + 1);
+            process(std::false_type{}, nullptr, c, xv, ~0ull, [&](int g) { xv[g] = load_elem<DIM>(pn + step * g); });
+        }
+        process(std::false_type{}, nullptr, kChunks - 1, xv, ~0ull, [](int) {});   // the last chunk loads nothing more
+    } else {
+        checked_chunks(std::false_type{}, nullptr);
+    }
+
+    // ---- the wave's header bytes (their positions do not depend on the
+    // offset); 2D/3D pad bytes [nblocks, H) are 0 -------------------------------
+    {
+        constexpr int NBW = kChunks * NB;
+        constexpr int BPL = (NBW + kWaveSize - 1) / kWaveSize;
+        static_assert(NBW <= 128, "header bytes of a wave");
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t wb0 = (DIM == 1) ? 2 * wgrp0 : wgrp0;
+#pragma unroll
+        for (int k = 0; k < BPL; ++k) {
+            const int j = BPL * lane + k;
+            const uint64_t b = wb0 + (uint64_t)j;
+            if (j < NBW && (DIM == 1 || b < geo.H)) dst[b] = sm.hdr[wave][j];
+        }
+    }
+
+    // ---- the tile's offset (wave 0) ---------------------------------------------
+    if (lane == 0) {
+        sm.wtot[wave] = wave_tot;
+        sm.over[wave] = over;
+        if (lossy) atomicOr(cur.err, kWarnLossy);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+        const uint64_t t = sm.wtot[w];
+        wexcl += (w < wave) ? t : 0;
+        ttot += t;
+    }
+    if (wave == 0) {
+        const uint64_t e = wave_lookback<LB>(sc, tile, cur.tile0, ttot, lane, [&](uint64_t p) {
+            return compress_tile_bytes<DIM, MODE, T, kChunks * G, WAVES>((const T*)cur.src, geo, r,
+                                                                        p - cur.tile0, lane);
+        });
+        if (lane == 0) {
+            sm.excl = e;
+            if (lt == geo.ntiles - 1 && cur.total) *cur.total = geo.H + e + ttot;
+        }
+    }
+    __syncthreads();
+    excl = sm.excl;
+    if (geo.H + excl + ttot > cur.limit) {   // never write past the caller's buffer
+        if (wave == 0 && lane == 0) atomicOr(cur.err, kErrOverflow);
         return;
     }
+    if (over) {   // pass 1: run the chunks again, storing directly
+        wave_tot = 0;
+        checked_chunks(std::true_type{}, dst + uniform64(geo.H + excl + wexcl));
+        return;
+    }
+
+    // ---- copy the staged payload: 16 bytes per lane, then the tail bytes ---------
+    uint8_t* const obase = dst + uniform64(geo.H + excl + wexcl);
+    typedef uint4 __attribute__((aligned(1))) u128u;
+    const uint32_t n16 = wave_tot & ~15u;
+    for (uint32_t i = 16u * (uint32_t)lane; i < n16; i += 16u * kWaveSize)
+        *(u128u*)(obase + i) = sm.stage[wave][i / 16];
+    if ((uint32_t)lane < (wave_tot & 15u)) obase[n16 + lane] = ((const uint8_t*)sm.stage[wave])[n16 + lane];
 }
 
 // Decompresses tile `tile` of array `cur` (every wave of the workgroup calls
