@@ -302,7 +302,7 @@ __device__ __forceinline__ uint32_t quantize(T x, T r) {
 // src/cuSZp_kernels_1D_f32.cu:1705-1711 for outlier, :837-843 otherwise)
 // ---------------------------------------------------------------------------
 struct BlockView {
-    uint32_t sel, nb, R, size, nwords, head;
+    uint32_t sel, nb, R, size, head;
 };
 template <int MODE, int S>
 __device__ __forceinline__ BlockView view_of(uint32_t byte) {
@@ -320,7 +320,6 @@ __device__ __forceinline__ BlockView view_of(uint32_t byte) {
         v.size = v.R ? S + S * v.R : 0;
         v.head = 0;
     }
-    v.nwords = v.size ? (S / 4) * (1 + v.R) : 0;   // sign + planes, in u32 words
     return v;
 }
 
@@ -337,6 +336,27 @@ __device__ __forceinline__ uint32_t rate_byte(uint32_t fr1, uint32_t fr2, uint32
         const uint32_t ofs2 = fr2 ? S + fr2 * S + nb : S + nb;
         return (ofs1 <= ofs2) ? fr1 : ((fr2 | 0x80u | ((nb - 1) << 5)) & 0xffu);
     }
+}
+
+// The outlier header byte and its decoder view in one step, for fr1, fr2 <= 31
+// (no block of 32 planes). There view_of(rate_byte(fr1, fr2, o_bits)) is the
+// encoder's own choice: a selected block has an element-0 residual, so nb is
+// 1..4 and the byte's fields [4:0] = fr2, [6:5] = nb - 1 decode back exactly;
+// an unselected byte fr1 <= 31 decodes to R = fr1 and size ofs1.
+template <int S>
+__device__ __forceinline__ BlockView outlier_code(uint32_t fr1, uint32_t fr2, uint32_t o_bits, uint32_t& byte) {
+    const uint32_t nb = (o_bits + 7) >> 3;
+    const uint32_t ofs1 = fr1 ? S + fr1 * S : 0;
+    const uint32_t ofs2 = S + fr2 * S + nb;
+    const bool sel = ofs1 > ofs2;
+    byte = sel ? (fr2 | 0x80u | ((nb - 1) << 5)) : fr1;
+    BlockView v;
+    v.sel = sel;
+    v.nb = nb;
+    v.R = sel ? fr2 : fr1;
+    v.size = sel ? ofs2 : ofs1;
+    v.head = sel ? nb : 0;
+    return v;
 }
 
 // Byte-granular little-endian store of 1..4 bytes (outlier prefixes).
@@ -1224,6 +1244,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                 // the words of all groups first, then their transposes side by
                 // side (one LDS round trip per stage for kXposeBatch groups),
                 // then the plane masks
+                uint32_t aor = 0;   // OR of the magnitudes: bit 31 set iff some block needs 32 planes
                 static_for<0, G>([&](auto gc) {
                     constexpr int g = decltype(gc)::value;
                     const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array (q = -1 if r = inf: masked below)
@@ -1231,7 +1252,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                     uint32_t d = residual(q);
                     d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
                     const uint32_t a = ((int32_t)d < 0) ? 0u - d : d;
-                    if constexpr (DIM != 1) anywide |= ballot((int32_t)a < 0);
+                    if constexpr (DIM != 1) aor |= a;
                     // sign bit over the low 31 magnitude bits; 1D ranks |-2^31| as 0
                     // (signed running maximum, src/cuSZp_kernels_1D_f32.cu:552), so bit 31 is free
                     W[g] = bfi(0x80000000u, d, a);
@@ -1244,6 +1265,7 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                         }
                     }
                 });
+                if constexpr (DIM != 1) anywide = ballot((int32_t)aor < 0);
                 constexpr int XB = kXposeBatch < G ? kXposeBatch : G;
 #pragma unroll
                 for (int g = 0; g < G; g += XB) xpose32_n<XB>(W + g, xc);
@@ -1251,7 +1273,12 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                     constexpr int g = decltype(gc)::value;
                     const uint64_t m1 = ballot(W[g] != 0) & 0x7fffffff7fffffffull;
                     uint64_t m2 = 0;
-                    if constexpr (MODE == kOutlier) m2 = ballot((W[g] & not_e0) != 0) & 0x7fffffff7fffffffull;
+                    if constexpr (MODE == kOutlier) {
+                        // planes nonzero without element 0 (bit 7 of the words of
+                        // planes 0..30; 2D/3D: low half only): W not in {0, 0x80}
+                        constexpr uint64_t e0lanes = (DIM == 1) ? 0x7fffffff7fffffffull : 0x7fffffffull;
+                        m2 = m1 & ~(ballot(W[g] == 0x80u) & e0lanes);
+                    }
                     if constexpr (DIM == 1) {
                         bk1 = wrlane<2 * g>((uint32_t)m1, bk1);
                         bk1 = wrlane<2 * g + 1>((uint32_t)(m1 >> 32), bk1);
@@ -1311,17 +1338,24 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
         // ---- bookkeeping: lane j <-> block j of this chunk ---------------------
         const bool blk = lane < NB;
         uint32_t byte;
-        if constexpr (MODE == kOutlier) byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
-        else byte = bitlen(bk1);
-        const BlockView v = view_of<MODE, S>(byte);
-        if constexpr (MODE == kOutlier && DIM != 1) {
-            // R1 = 32 (a residual of magnitude 2^31, only from saturated or
-            // wrapped codes) is kept as the plain byte 32, which every
-            // outlier decoder reads as rate 0 (written by
-            // src/cuSZp_kernels_2D_f32.cu:1494-1505, masked with 0x1f by the
-            // decoder at :2320): the block decodes to zeros, so the call
-            // reports it
-            lossy |= ballot(blk && byte == 32u) != 0;
+        BlockView v;
+        if constexpr (MODE == kOutlier) {
+            if (DIM == 1 || !anywide) {
+                v = outlier_code<S>(bitlen(bk1), bitlen(bk2), bitlen(bko), byte);
+            } else {
+                byte = rate_byte<MODE, S>(bitlen(bk1), bitlen(bk2), bitlen(bko));
+                v = view_of<MODE, S>(byte);
+                // R1 = 32 (a residual of magnitude 2^31, only from saturated or
+                // wrapped codes) is kept as the plain byte 32, which every
+                // outlier decoder reads as rate 0 (written by
+                // src/cuSZp_kernels_2D_f32.cu:1494-1505, masked with 0x1f by the
+                // decoder at :2320): the block decodes to zeros, so the call
+                // reports it
+                lossy |= ballot(blk && byte == 32u) != 0;
+            }
+        } else {
+            byte = bitlen(bk1);
+            v = view_of<MODE, S>(byte);
         }
         const uint32_t sz = blk ? v.size : 0u;
         const uint32_t incl = scan_row16(sz);   // the chunk's NB <= 16 blocks sit in lanes 0..NB-1
