@@ -438,9 +438,25 @@ __device__ __forceinline__ bool status_ready(uint64_t w, uint64_t tag) {
 // held only by a running wave that computes without waiting, so every wait
 // ends; the owner's later store of the same aggregate (or of its inclusive
 // prefix) is harmless.
+//
+// pre (with has_pre): the window's first read, issued earlier by the caller
+// with lookback_window(); its words may be stale (pending ones are read again).
+template <int K>
+struct LookbackWindow {
+    uint64_t w[K];
+};
+template <int K>
+__device__ __forceinline__ LookbackWindow<K> lookback_window(const ScanCtx& sc, uint64_t tile, int lane) {
+    LookbackWindow<K> pw;
+    const unsigned long long* const wp = sc.status + ((int64_t)tile - 1 - lane);
+#pragma unroll
+    for (int k = 0; k < K; ++k) pw.w[k] = ld_status(wp - 64 * k);
+    return pw;
+}
 template <int K, class AggF>
 __device__ __forceinline__ uint64_t wave_lookback(const ScanCtx& sc, uint64_t tile, uint64_t first, uint64_t agg,
-                                                  int lane, AggF&& agg_of) {
+                                                  int lane, AggF&& agg_of, const LookbackWindow<K> pre = {},
+                                                  bool has_pre = false) {
     constexpr int WIN = 64 * K;
     const uint64_t tag = sc.tag;
     if (sc.hold_every && tile % sc.hold_every == sc.hold_every - 1) {
@@ -473,8 +489,14 @@ __device__ __forceinline__ uint64_t wave_lookback(const ScanCtx& sc, uint64_t ti
         const unsigned long long* const wp = sc.status + (hi - 1 - lane);   // position 64k + lane: wp - 64k
         const int64_t rel = hi - 1 - lane - (int64_t)first;                 // before the segment iff rel < 64k
         uint64_t w[K];
+        if (has_pre) {   // the first window, read before (only once: a claim re-reads)
 #pragma unroll
-        for (int k = 0; k < K; ++k) w[k] = ld_status(wp - 64 * k);
+            for (int k = 0; k < K; ++k) w[k] = pre.w[k];
+            has_pre = false;
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) w[k] = ld_status(wp - 64 * k);
+        }
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (rel < 64 * k) w[k] = tag | kFlagPrefix;
@@ -958,6 +980,9 @@ unit_offsets_kernel(const uint8_t* __restrict__ cmp, const uint64_t limit, const
     for (int k = 0; k < UPT; ++k) tot += sums[k];
     const uint32_t incl = scan_wave(tot);   // < 2^32: a segment holds at most 2^15 blocks of <= 2 KiB
     if (lane == kWaveSize - 1) wsum[wave] = incl;
+    // wave 0's first lookback read overlaps the other waves' sums
+    LookbackWindow<kLookbackWords> pre = {};
+    if (wave == 0 && seg != 0) pre = lookback_window<kLookbackWords>(sc, seg, lane);
     __syncthreads();
     uint32_t wex = 0, stot = 0;
 #pragma unroll
@@ -969,7 +994,7 @@ unit_offsets_kernel(const uint8_t* __restrict__ cmp, const uint64_t limit, const
     if (wave == 0) {
         const uint64_t e = wave_lookback<kLookbackWords>(sc, seg, 0, stot, lane, [&](uint64_t p) {
             return segment_bytes<MODE, S>(cmp, limit, p, lane);
-        });
+        }, pre, true);
         if (lane == 0) {
             excl = e;
             if (seg == nseg - 1 && total) *total = H + e + stot;
