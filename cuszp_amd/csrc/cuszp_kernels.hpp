@@ -1051,6 +1051,12 @@ constexpr int kOutStage = 16 * (128 + kOutPad);   // the larger of 16 x 128 B an
 // into the LDS buffer of a decompress wave; kOutStage bytes hold it.
 constexpr int kDenseBytes = 2048;
 static_assert(kDenseBytes <= kOutStage, "dense chunk buffer");
+// The LDS buffer of a decompress wave: a 1D wave's whole payload, read in one
+// burst of up to four 16-byte-per-lane loads where it fits; otherwise one
+// chunk's payload at a time (kDenseBytes), or 2D/3D decoded rows (kOutStage).
+// 4032 bytes keep four 8-wave workgroups within a CU's 160 KiB of LDS.
+constexpr int kWaveBuf = 4032;
+static_assert(kWaveBuf >= kOutStage && kWaveBuf % 16 == 0, "wave buffer");
 
 // Shared memory of a compress workgroup.
 template <int WAVES>
@@ -1101,9 +1107,10 @@ struct SmemD {
     uint64_t wtot[WAVES];             // per-wave byte counts
     uint64_t excl;                    // the tile's exclusive prefix (wave 0's lookback)
     DMeta meta[WAVES][kMetaPerWave];  // per-block metadata of the wave's blocks
-    // a chunk's stream words, loaded densely (then read per lane); 2D/3D: after
-    // that the chunk's decoded rows, padded
-    uint4 ostage[WAVES][kOutStage / 16];
+    // the wave's stream words, loaded densely (then read per lane): its whole
+    // payload (1D, where it fits) or one chunk's; 2D/3D: after that the
+    // chunk's decoded rows, padded
+    uint4 ostage[WAVES][kWaveBuf / 16];
     uint32_t zero[WAVES][2];          // two zero words per wave (read by lanes without a word)
 };
 
@@ -1848,10 +1855,54 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
         // payload's last bytes must lie inside it: an outlier stream whose length
         // is not a multiple of 4 ends inside a dword, and its last wave keeps the
         // per-lane loads (a truncated stream has an empty range: the same).
-        bool dense = ((wtot + 3u) & ~3u) <= (uint32_t)nrec;
+        const bool whole = ((wtot + 3u) & ~3u) <= (uint32_t)nrec;   // every payload dword inside the range
+        bool dense = whole;
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) dense &= cs[c + 1] - (cs[c] & ~3u) <= (uint32_t)kDenseBytes;
-        if (dense) {
+        // 1D: the wave's whole payload in one burst of loads before its first
+        // chunk, where it fits the wave's buffer. A compute-free kernel with the
+        // 1D decode's traffic (2^30 f32 written, 0.94 GB read) ran 2.5-4 % faster
+        // with each wave's reads in one burst than with a read per chunk.
+        const bool burst = DIM == 1 && whole && wtot <= (uint32_t)kWaveBuf;
+        if (burst) {
+            typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+            constexpr int NL = (kWaveBuf + 1023) / 1024;
+            lds_u8* const buf = (lds_u8*)sm.ostage[wave];
+            if (lane < 2) sm.zero[wave][lane] = 0;
+            const lds_u8* const zw = (const lds_u8*)&sm.zero[wave][0];
+            v4u P[NL];
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                // 1 KiB per load; none past the payload; lanes past the buffer read nothing
+                const uint32_t o = 1024u * (uint32_t)k + 16u * (uint32_t)lane;
+                P[k] = __builtin_amdgcn_raw_buffer_load_b128(
+                    rsrc, (1024u * (uint32_t)k < wtot && o < (uint32_t)kWaveBuf) ? o : kAbsent, 0, 0);
+            }
+            prefixes_ready();
+#pragma unroll
+            for (int k = 0; k < NL; ++k)
+                if (1024 * k + 16 * lane < kWaveBuf) ((__attribute__((address_space(3))) v4u*)buf)[64 * k + lane] = P[k];
+            __builtin_amdgcn_wave_barrier();
+            // this lane's word of group g of chunk c (block offsets are payload offsets)
+            auto word_buf = [&](int c, int g) -> uint32_t {
+                const DMeta m = meta(c, g);
+                typedef __attribute__((address_space(3))) const uint32_t lds_cu32;
+                if constexpr (MODE != kOutlier) {
+                    return *(lds_cu32*)(((m.nb >> l) & 1) ? zw : buf + (m.a + wo));
+                } else {
+                    const uint32_t o = (m.a & 0xfffffu) + wo;
+                    lds_cu32* const q = (lds_cu32*)(((m.nb >> l) & 1) ? zw : buf + (o & ~3u));
+                    return __builtin_amdgcn_alignbyte(q[1], q[0], o & 3u);
+                }
+            };
+            static_for<0, kChunks>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                uint32_t W[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) W[g] = word_buf(c, g);
+                decode_chunk(c, W);
+            });
+        } else if (dense) {
             typedef uint32_t v4u __attribute__((ext_vector_type(4)));
             lds_u8* const buf = (lds_u8*)sm.ostage[wave];
             if (lane < 2) sm.zero[wave][lane] = 0;
