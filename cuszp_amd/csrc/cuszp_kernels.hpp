@@ -702,8 +702,9 @@ __device__ __forceinline__ uint32_t inverse_lorenzo(uint32_t d, int e) {
         vy += dpp<kDppRowShr4>(vy);
         vy += dpp<kDppRowShr8>(vy);
         vy = dpp<kDppQuadBcast0>(vy);
-        uint32_t vz = dpp<kDppRowNewBcast0>((x == 0 && y == 0) ? d : 0u);   // the row's first lane, across the row
-        vz += dpp<kDppBcast15, 0xa>(vz);   // prefix over the rows
+        // the row's first lane (x = y = 0) across the row, then the prefix over the rows
+        uint32_t vz = dpp<kDppRowNewBcast0>(d);
+        vz += dpp<kDppBcast15, 0xa>(vz);
         vz += dpp<kDppBcast31, 0xc>(vz);
         return vz + vy + vx;
     }
@@ -1073,32 +1074,37 @@ struct SmemC {
 // Shared memory of a decompress workgroup.
 // Per-block metadata of a decompress wave, in LDS (one 8-byte entry per block
 // of the wave, written once by the wave's bookkeeping):
-//   a:  [19:0] byte offset of the block's sign word within the wave's payload
-//       (past any outlier prefix), [30] 32 planes (2D/3D only: the sign words
-//       move in front of the planes), [31] outlier-selected
-//   nb: bit l CLEAR when lane l of a 32-lane half carries one of the block's
-//       stream words (planes 0..R-1 in lanes 0..R-1, the sign word in lane 31)
+//   a:  byte offset of the block's sign word within the wave's payload (past any
+//       outlier prefix)
+//   nb: [31:24] 0 for a block without payload (or of a truncated stream), else
+//       R + 1 (33 for 32 planes); [8] outlier-selected; [9] 32 planes (2D/3D
+//       only: the sign words move in front of the planes)
+// Lane l of a 32-lane half carries one of the block's words iff key(l) < R + 1,
+// with key(31) = 0 (the sign word, or plane 31 of a 32-plane block) and
+// key(l) = l + 1 (plane l). A lane holds its key as (key << 24) | 0xffffff, so
+// the test is one unsigned compare with nb whatever its low flag bits.
 struct DMeta {
     uint32_t a, nb;
-    __device__ __forceinline__ uint32_t off() const { return a & 0xfffffu; }
-    __device__ __forceinline__ bool wide() const { return (a >> 30) & 1; }
-    __device__ __forceinline__ bool sel() const { return (int32_t)a < 0; }
-    __device__ __forceinline__ bool nz() const { return nb != ~0u; }
-    // Buffer offset of lane l's word, wo bytes into the block (a block without
-    // 32 planes): bits 30..0 from the offset, bit 31 set for a lane without a
-    // word, i.e. past every buffer range (reads 0). Three VALU operations.
-    __device__ __forceinline__ uint32_t word_at(uint32_t l, uint32_t wo) const {
-        return bfi(0x7fffffffu, a + wo, nb << (31u - l));
+    __device__ __forceinline__ uint32_t off() const { return a; }
+    __device__ __forceinline__ bool wide() const { return (nb >> 9) & 1; }
+    __device__ __forceinline__ bool sel() const { return (nb >> 8) & 1; }
+    __device__ __forceinline__ bool nz() const { return nb >= (1u << 24); }
+    __device__ __forceinline__ bool has(uint32_t keyx) const { return keyx < nb; }
+    // Buffer offset of the lane's word, wo bytes into the block (a block
+    // without 32 planes), or an offset past every buffer range (reads 0).
+    __device__ __forceinline__ uint32_t word_at(uint32_t keyx, uint32_t wo) const {
+        return has(keyx) ? a + wo : kAbsent;
     }
 };
 template <int DIM>
 __device__ __forceinline__ DMeta make_dmeta(uint32_t off, const BlockView& v, bool nz) {
     DMeta m;
     const bool wide = DIM != 1 && v.R >= 32;
-    m.a = (off + v.head) | ((uint32_t)wide << 30) | (v.sel << 31);
-    m.nb = ~(!nz ? 0u : v.R >= 32 ? ~0u : (((1u << v.R) - 1u) | 0x80000000u));
+    m.a = off + v.head;
+    m.nb = (nz ? (v.R + 1u) << 24 : 0u) | (v.sel << 8) | ((uint32_t)wide << 9);
     return m;
 }
+__device__ __forceinline__ uint32_t lane_keyx(uint32_t l) { return (((l == 31) ? 0u : l + 1u) << 24) | 0xffffffu; }
 
 // Blocks of metadata per decompress wave (its kChunks * blocks-per-chunk).
 constexpr int kMetaPerWave = 128;
@@ -1702,6 +1708,7 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     const T eb = (T)cur.param;
     const uint32_t wo = word_offset<DIM>(l, h, true);
     const uint32_t wo_wide = word_offset<DIM>(l, h, false);
+    const uint32_t keyx = lane_keyx(l);   // see DMeta
     constexpr uint64_t step = (DIM == 1) ? 64 : (DIM == 2 ? 8 : 4);
     // fast tile: every element inside the array and (2D/3D) each chunk's blocks
     // consecutive along x
@@ -1712,14 +1719,14 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
     // stream word of this lane in group g of chunk c (0 for a lane without a
     // word and for every lane of a truncated stream); no 32-plane blocks
     auto word = [&](int c, int g) -> uint32_t {
-        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, meta(c, g).word_at(l, wo), 0, 0);
+        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, meta(c, g).word_at(keyx, wo), 0, 0);
     };
     // the same where 2D/3D blocks of 32 planes may occur
     auto word_any = [&](int c, int g) -> uint32_t {
         const DMeta m = meta(c, g);
         const bool wideg = (DIM != 1) && (MODE != kOutlier) && m.wide();
         const uint32_t o = m.off() + (wideg ? wo_wide : wo);
-        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, ((m.nb >> l) & 1) ? kAbsent : o, 0, 0);
+        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, m.has(keyx) ? o : kAbsent, 0, 0);
     };
     // residuals of group g of chunk c from its word after the transpose (lane
     // e: element e)
@@ -1888,11 +1895,12 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
                 const DMeta m = meta(c, g);
                 typedef __attribute__((address_space(3))) const uint32_t lds_cu32;
                 if constexpr (MODE != kOutlier) {
-                    return *(lds_cu32*)(((m.nb >> l) & 1) ? zw : buf + (m.a + wo));
+                    return *(lds_cu32*)(m.has(keyx) ? buf + (m.a + wo) : zw);
                 } else {
-                    const uint32_t o = (m.a & 0xfffffu) + wo;
-                    lds_cu32* const q = (lds_cu32*)(((m.nb >> l) & 1) ? zw : buf + (o & ~3u));
-                    return __builtin_amdgcn_alignbyte(q[1], q[0], o & 3u);
+                    // (v_alignbyte uses the low two bits of its shift operand)
+                    const uint32_t o = m.a + wo;
+                    lds_cu32* const q = (lds_cu32*)(m.has(keyx) ? buf + (o & ~3u) : zw);
+                    return __builtin_amdgcn_alignbyte(q[1], q[0], o);
                 }
             };
             static_for<0, kChunks>([&](auto cc) {
@@ -1927,13 +1935,14 @@ __device__ __forceinline__ void decompress_tile(const Item& cur, const uint64_t 
                 if constexpr (MODE != kOutlier) {
                     // fixed/plain blocks are whole 4-byte words: an aligned read
                     const uint32_t o = m.a + wo - (cs[c] & ~3u);
-                    return *(lds_cu32*)(((m.nb >> l) & 1) ? zw : buf + o);
+                    return *(lds_cu32*)(m.has(keyx) ? buf + o : zw);
                 } else {
                     // outlier prefixes shift later blocks by 1-3 bytes: the two
                     // aligned words around the word, then a byte funnel shift
-                    const uint32_t o = (m.a & 0xfffffu) + wo - (cs[c] & ~3u);
-                    lds_cu32* const q = (lds_cu32*)(((m.nb >> l) & 1) ? zw : buf + (o & ~3u));
-                    return __builtin_amdgcn_alignbyte(q[1], q[0], o & 3u);
+                    // (v_alignbyte uses the low two bits of its shift operand)
+                    const uint32_t o = m.a + wo - (cs[c] & ~3u);
+                    lds_cu32* const q = (lds_cu32*)(m.has(keyx) ? buf + (o & ~3u) : zw);
+                    return __builtin_amdgcn_alignbyte(q[1], q[0], o);
                 }
             };
             v4u PA[2], PB[2];
