@@ -151,6 +151,7 @@ inline Item make_item(const void* src, void* dst, uint64_t limit, unsigned long 
     it.param = param;
     it.tile0 = 0;
     it.variant = (uint32_t)(type * 9 + (dim - 1) * 3 + mode);
+    it.flags = g.n * (type == CUSZP_TYPE_DOUBLE ? 8u : 4u) >= kStreamNtBytes ? kItemStreamNt : 0u;
     it.geo = g;
     return it;
 }

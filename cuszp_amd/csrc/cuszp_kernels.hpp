@@ -66,6 +66,14 @@ constexpr uint32_t kInfoStolen = 8u;     // a waiting wave computed a predecesso
 constexpr uint32_t kWarnLossy = 16u;     // a 2D/3D outlier block has rate byte 32, which decoders read as rate 0
 constexpr uint32_t kErrMask = 7u;
 
+// Item flags. kItemStreamNt: compress writes the stream with nontemporal
+// stores. Set for arrays of at least kStreamNtBytes: their streams outgrow the
+// memory-side cache at typical ratios, and dirty stream lines left there
+// would be written back during the next kernel. A smaller stream is stored
+// normally and stays cached for its next reader.
+constexpr uint32_t kItemStreamNt = 1u;
+constexpr uint64_t kStreamNtBytes = 1ull << 30;
+
 // Bound on lookback polls (each polls the status words once): far above any
 // legitimate wait, it guarantees that every wave of a launch exits even if a
 // predecessor tile never publishes.
@@ -110,7 +118,7 @@ struct Item {
     double param;                   // compress: 0.5/eb, decompress: eb (exact in the array's type)
     uint64_t tile0;                 // first tile of the array in the launch (its scan segment start)
     uint32_t variant;               // type * 9 + (dim - 1) * 3 + mode
-    uint32_t pad_;
+    uint32_t flags;                 // kItemStreamNt
     Geom geo;                       // geo.ntiles: tiles of this array
 };
 
@@ -1564,10 +1572,16 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
 
     // ---- copy the staged payload: 16 bytes per lane, then the tail bytes ---------
     uint8_t* const obase = dst + uniform64(geo.H + excl + wexcl);
-    typedef uint4 __attribute__((aligned(1))) u128u;
+    typedef uint32_t v4uu __attribute__((ext_vector_type(4), aligned(1)));
+    typedef uint32_t v4ul __attribute__((ext_vector_type(4)));
     const uint32_t n16 = wave_tot & ~15u;
-    for (uint32_t i = 16u * (uint32_t)lane; i < n16; i += 16u * kWaveSize)
-        *(u128u*)(obase + i) = sm.stage[wave][i / 16];
+    if (cur.flags & kItemStreamNt) {
+        for (uint32_t i = 16u * (uint32_t)lane; i < n16; i += 16u * kWaveSize)
+            __builtin_nontemporal_store(*(const v4ul*)&sm.stage[wave][i / 16], (v4uu*)(obase + i));
+    } else {
+        for (uint32_t i = 16u * (uint32_t)lane; i < n16; i += 16u * kWaveSize)
+            *(v4uu*)(obase + i) = *(const v4ul*)&sm.stage[wave][i / 16];
+    }
     if ((uint32_t)lane < (wave_tot & 15u)) obase[n16 + lane] = ((const uint8_t*)sm.stage[wave])[n16 + lane];
 }
 
