@@ -1291,11 +1291,8 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                 // side (one LDS round trip per stage for kXposeBatch groups),
                 // then the plane masks
                 uint32_t aor = 0;   // OR of the magnitudes: bit 31 set iff some block needs 32 planes
-                static_for<0, G>([&](auto gc) {
+                auto code = [&](auto gc, uint32_t d) {   // d: group g's residual
                     constexpr int g = decltype(gc)::value;
-                    const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array (q = -1 if r = inf: masked below)
-                    roll_fn(g);
-                    uint32_t d = residual(q);
                     d = ((vm >> g) & 1) ? d : 0u;   // codes and residuals of padding are 0
                     const uint32_t a = ((int32_t)d < 0) ? 0u - d : d;
                     if constexpr (DIM != 1) aor |= a;
@@ -1310,7 +1307,33 @@ __device__ __forceinline__ void compress_tile(const Item& cur, const uint64_t ti
                             bko = wrlane<g>(rdlane(a, 0), bko);
                         }
                     }
-                });
+                };
+                if constexpr (DIM == 1 || MODE == kFixed) {
+                    static_for<0, G>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        const uint32_t q = quantize<T>(xval(g), r);   // x = 0 outside the array (q = -1 if r = inf: masked below)
+                        roll_fn(g);
+                        code(gc, residual(q));
+                    });
+                } else {
+                    // 2D/3D Lorenzo: every group's codes first (W holds them), then
+                    // the predecessors' codes of all groups in flight together (one
+                    // LDS permute each), then the residuals
+                    static_for<0, G>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        W[g] = quantize<T>(xval(g), r);
+                        roll_fn(g);
+                    });
+                    uint32_t QP[G];
+                    static_for<0, G>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        QP[g] = (uint32_t)__builtin_amdgcn_ds_bpermute(pl_src << 2, (int)W[g]);
+                    });
+                    static_for<0, G>([&](auto gc) {
+                        constexpr int g = decltype(gc)::value;
+                        code(gc, W[g] - ((pl >= 0) ? QP[g] : 0u));
+                    });
+                }
                 if constexpr (DIM != 1) anywide = ballot((int32_t)aor < 0);
                 constexpr int XB = kXposeBatch < G ? kXposeBatch : G;
 #pragma unroll
